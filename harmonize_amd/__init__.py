@@ -1,0 +1,229 @@
+"""Python handle on libharmonize_amd.so, the gfx950 asynchronous execution runtime.
+
+The runtime itself is C++/HIP behind the C ABI declared in include/harmonize_amd.h;
+this module only loads that library with ctypes and wraps its entry points for
+tests and benchmarks. There is no CPU fallback: if the library is missing or
+cannot be loaded, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libharmonize_amd.so"
+
+EVENT = 0
+ASYNC = 1
+PLAIN = 2
+KINDS = {"event": EVENT, "async": ASYNC, "plain": PLAIN}
+
+COLLATZ_OVERFLOW = 0xFFFFFFFF
+
+
+class HarmonizeError(RuntimeError):
+    """A failure reported by the runtime library."""
+
+
+class RunConfig(ctypes.Structure):
+    _fields_ = [
+        ("groups", ctypes.c_uint32),
+        ("cycles", ctypes.c_uint32),
+        ("arena", ctypes.c_uint64),
+        ("queues", ctypes.c_uint32),
+        ("coalesce", ctypes.c_uint32),
+        ("max_launches", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("promises", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("spills", ctypes.c_uint64),
+        ("fills", ctypes.c_uint64),
+        ("make_work_calls", ctypes.c_uint64),
+        ("groups", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("elapsed_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class DeviceFacts(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("compute_units", ctypes.c_int32),
+        ("max_groups_collatz_async", ctypes.c_int32),
+        ("max_groups_collatz_event", ctypes.c_int32),
+        ("total_memory", ctypes.c_int64),
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 64),
+    ]
+
+
+_lib = None
+
+# C signatures of the entry points used from Python: name -> (restype, argtypes).
+_VP = ctypes.c_void_p
+_SIGNATURES = {
+    "hz_last_error": (ctypes.c_char_p, []),
+    "harmonize_alloc_device_bytes": (_VP, [ctypes.c_size_t]),
+    "harmonize_alloc_managed_bytes": (_VP, [ctypes.c_size_t]),
+    "harmonize_free_device_bytes": (None, [_VP]),
+    "harmonize_memcpy_host_to_device": (None, [_VP, _VP, ctypes.c_size_t]),
+    "harmonize_memcpy_device_to_host": (None, [_VP, _VP, ctypes.c_size_t]),
+    "hz_device_synchronize": (ctypes.c_int, []),
+    "hz_device_query": (ctypes.c_int, [ctypes.POINTER(DeviceFacts)]),
+    "hz_collatz_run": (
+        ctypes.c_int,
+        [ctypes.c_int, _VP, ctypes.c_uint32, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
+    ),
+    "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+}
+
+PROGRAM_SUFFIXES = ("collatz_event", "collatz_async")
+for _s in PROGRAM_SUFFIXES:
+    _SIGNATURES.update(
+        {
+            f"alloc_state_{_s}": (_VP, []),
+            f"free_state_{_s}": (None, [_VP]),
+            f"load_state_device_{_s}": (None, [_VP, _VP]),
+            f"store_state_device_{_s}": (None, [_VP, _VP]),
+            f"alloc_program_{_s}": (_VP, [_VP, ctypes.c_size_t]),
+            f"free_program_{_s}": (None, [_VP]),
+            f"init_program_{_s}": (None, [_VP, ctypes.c_size_t]),
+            f"exec_program_{_s}": (None, [_VP, ctypes.c_size_t, ctypes.c_size_t]),
+            f"complete_{_s}": (ctypes.c_int, [_VP]),
+            f"clear_flags_{_s}": (None, [_VP]),
+            f"set_device_{_s}": (None, [ctypes.c_int]),
+        }
+    )
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded runtime library (raises HarmonizeError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HarmonizeError(
+                f"{LIB_PATH} is not built; run `python -m harmonize_amd.build` (hipcc, gfx950)"
+            )
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str | None:
+    msg = lib().hz_last_error()
+    return msg.decode() if msg else None
+
+
+def check(status: int, what: str = "call") -> None:
+    if status != 0:
+        raise HarmonizeError(f"{what} failed ({status}): {last_error()}")
+
+
+def check_void(what: str) -> None:
+    """For the reference-shaped void entry points: raise if they recorded an error."""
+    msg = last_error()
+    if msg:
+        raise HarmonizeError(f"{what} failed: {msg}")
+
+
+class DeviceArray:
+    """A device allocation holding a 1-D numpy-typed array."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = max(1, self.count * self.dtype.itemsize)
+        ptr = lib().harmonize_alloc_device_bytes(self.nbytes)
+        if not ptr:
+            raise HarmonizeError(f"device allocation of {self.nbytes} bytes failed: {last_error()}")
+        self.ptr = ptr
+
+    @classmethod
+    def from_numpy(cls, host: np.ndarray) -> "DeviceArray":
+        host = np.ascontiguousarray(host)
+        arr = cls(host.size, host.dtype)
+        arr.upload(host)
+        return arr
+
+    def upload(self, host: np.ndarray) -> None:
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if host.size != self.count:
+            raise ValueError("size mismatch")
+        if self.count:
+            lib().harmonize_memcpy_host_to_device(self.ptr, host.ctypes.data, host.nbytes)
+            check_void("upload")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        if self.count:
+            lib().harmonize_memcpy_device_to_host(out.ctypes.data, self.ptr, out.nbytes)
+            check_void("download")
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            lib().harmonize_free_device_bytes(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_facts() -> dict:
+    f = DeviceFacts()
+    check(lib().hz_device_query(ctypes.byref(f)), "hz_device_query")
+    return {
+        "device": f.device,
+        "compute_units": f.compute_units,
+        "max_groups_collatz_async": f.max_groups_collatz_async,
+        "max_groups_collatz_event": f.max_groups_collatz_event,
+        "total_memory": f.total_memory,
+        "name": f.name.decode(),
+        "arch": f.arch.decode(),
+    }
+
+
+def make_config(**kw) -> RunConfig:
+    cfg = RunConfig()
+    for k, v in kw.items():
+        setattr(cfg, k, int(v))
+    return cfg
+
+
+def collatz_run(kind, steps: DeviceArray, starts: DeviceArray | None = None, count: int | None = None, **cfg) -> dict:
+    """Runs the Collatz program on the GPU; returns its statistics."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    n = steps.count if count is None else int(count)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(
+        lib().hz_collatz_run(k, starts.ptr if starts is not None else None, n, steps.ptr, ctypes.byref(config),
+                             ctypes.byref(st)),
+        "hz_collatz_run",
+    )
+    return st.as_dict()
+
+
+def collatz_make_starts(seed: int, bits: int, count: int) -> DeviceArray:
+    out = DeviceArray(count, np.uint64)
+    check(lib().hz_collatz_make_starts(seed, bits, count, out.ptr), "hz_collatz_make_starts")
+    return out

@@ -1,0 +1,868 @@
+// AsyncProgram<Spec>: the persistent, promise-coalescing program type.
+//
+// Semantics follow the reference's program/async.h: a persistent kernel in
+// which every group repeatedly picks a link of same-operation promises and
+// evaluates it one promise per lane (exec_cycle :2155-2319), calls the
+// spec's make_work when it has no full link (:2201-2222), spills surplus
+// links to device memory and refills from it (:1246-1308, :1800-2010), and
+// the last group to leave decides completion (cleanup_runtime :2323-2393).
+// A launch that ends early (cycle budget, halt_early) leaves every promise in
+// device memory so that the next exec resumes it.
+//
+// The design is gfx950-native rather than a translation:
+//  - a group is one 64-lane wavefront and a link holds 64 promises;
+//  - coalescing uses the EXEC mask: one ballot, mbcnt lane ranks and
+//    readfirstlane broadcasts per call site; the stash bookkeeping is
+//    private to the wavefront, so it needs no atomics and no barriers;
+//  - stash links live in LDS in structure-of-arrays form, [link][word][lane],
+//    so appends, reads, spills and refills are bank-conflict free and spills
+//    and refills move whole 256-byte rows with all 64 lanes (the reference
+//    copies links with a single lane and re-enqueues refilled promises one at
+//    a time, async.h:1207, :1767-1771);
+//  - the HBM arena holds links in the same [link][word][lane] layout;
+//    full links travel through bounded lock-free ring queues of link ids
+//    (one queue per XCD by default, selected from blockIdx % 8), with
+//    stealing from other queues when a group's own queue is empty;
+//  - termination is counter-based: `units` counts groups that hold promises
+//    plus links parked in queues, `makers` counts groups whose make_work may
+//    still produce work; the run is quiescent when both are zero.
+#pragma once
+
+#include "host.h"
+#include "iter.h"
+#include "spec.h"
+
+namespace hz {
+
+// One bounded multi-producer multi-consumer ring of link ids. The three
+// counters sit on separate 128-byte lines.
+struct LinkQueue {
+    unsigned int tail;
+    unsigned int pad0[31];
+    unsigned int head;
+    unsigned int pad1[31];
+    int count;
+    unsigned int pad2[31];
+};
+
+// Run status and statistics of an AsyncProgram instance, kept in HBM.
+struct AsyncStatus {
+    unsigned int flags;
+    unsigned int pad0[31];
+    int units;
+    unsigned int pad1[31];
+    int makers;
+    unsigned int pad2[31];
+    unsigned int checkout;
+    unsigned int claim;
+    unsigned int pad3[30];
+    unsigned long long int promises;
+    unsigned long long int spills;
+    unsigned long long int fills;
+    unsigned long long int make_work_calls;
+    unsigned long long int launches;
+    unsigned long long int idle_polls;
+};
+
+constexpr unsigned int LINK_NONE = 0xFFFFFFFFu;
+
+}  // namespace hz
+
+template <typename PROGRAM_SPEC>
+class AsyncProgram {
+   public:
+    typedef AsyncProgram<PROGRAM_SPEC> ProgramType;
+
+    using AdrType = typename hz::spec::AdrTypeOr<PROGRAM_SPEC, unsigned int>::type;
+    using OpSet = typename hz::spec::OpSetOr<PROGRAM_SPEC, OpUnion<>>::type;
+    using DeviceState = typename hz::spec::DeviceStateOr<PROGRAM_SPEC, VoidState>::type;
+    using GroupState = typename hz::spec::GroupStateOr<PROGRAM_SPEC, VoidState>::type;
+    using ThreadState = typename hz::spec::ThreadStateOr<PROGRAM_SPEC, VoidState>::type;
+
+    template <typename OP_SET, typename ADR_TYPE>
+    friend struct RemappingBarrier;
+
+    typedef PromiseUnion<OpSet> PromiseUnionType;
+
+    template <typename TYPE>
+    struct Lookup {
+        typedef typename PromiseUnionType::template Lookup<TYPE>::type type;
+    };
+
+    static_assert(!hz::spec::GROUP_SIZEOr<PROGRAM_SPEC, 64>::given ||
+                      hz::spec::GROUP_SIZEOr<PROGRAM_SPEC, 64>::value == 64,
+                  "On gfx950 a group is one 64-lane wavefront; GROUP_SIZE must be 64 if given.");
+
+    static const size_t STASH_SIZE = hz::spec::STASH_SIZEOr<PROGRAM_SPEC, 16>::value;
+    static const size_t FRAME_SIZE = hz::spec::FRAME_SIZEOr<PROGRAM_SPEC, 32>::value;
+    static const size_t POOL_SIZE = hz::spec::POOL_SIZEOr<PROGRAM_SPEC, 32>::value;
+    static const size_t STACK_SIZE = hz::spec::STACK_SIZEOr<PROGRAM_SPEC, 0>::value;
+    static const size_t GROUP_SIZE = 64;
+    static const size_t WORK_GROUP_SIZE = 64;
+
+    static_assert(STACK_SIZE == 0, "only a flat work stack is supported (as in the reference)");
+
+    static const unsigned char FN_ID_COUNT = PromiseUnionType::Info::COUNT;
+    static constexpr uint32_t OPS = PromiseUnionType::Info::COUNT;
+    static constexpr uint32_t WORDS = hz::PromiseWords<OpSet>::value;
+
+    static_assert(std::is_trivially_copyable<PromiseUnionType>::value,
+                  "promise arguments must be trivially copyable");
+
+    // LDS stash: STASH_SIZE links of 64 promises, bounded to 32 links and to
+    // 48 KiB per wavefront, and never fewer than one open link per operation
+    // plus headroom.
+    static constexpr uint32_t LINK_BYTES = WORDS * 64u * 4u;
+    static constexpr uint32_t LDS_LINK_CAP = (48u * 1024u) / LINK_BYTES;
+    static constexpr uint32_t MIN_LINKS = OPS + 3u;
+    static constexpr uint32_t WANT_LINKS =
+        (STASH_SIZE < 32 ? (uint32_t)STASH_SIZE : 32u) < LDS_LINK_CAP ? (STASH_SIZE < 32 ? (uint32_t)STASH_SIZE : 32u)
+                                                                      : LDS_LINK_CAP;
+    static constexpr uint32_t LINKS = WANT_LINKS > MIN_LINKS ? WANT_LINKS : MIN_LINKS;
+    static_assert(LINKS <= 32, "operation set too large for one wavefront's stash");
+    static constexpr uint32_t ALL_LINKS = (LINKS == 32) ? 0xFFFFFFFFu : ((1u << LINKS) - 1u);
+    // Free links kept before a link is evaluated: one per operation for the
+    // links its children may open, plus one.
+    static constexpr uint32_t MARGIN = OPS + 1u;
+    static constexpr uint32_t SPARES = 4;
+
+    static const unsigned int BAD_FUNC_ID_FLAG = 0x00000001;
+    static const unsigned int STASH_FAIL_FLAG = 0x00000002;
+    static const unsigned int ARENA_EXHAUSTED_FLAG = 0x00000008;
+    static const unsigned int QUEUE_STALL_FLAG = 0x00000010;
+    static const unsigned int EARLY_HALT_FLAG = 0x40000000;
+    static const unsigned int COMPLETION_FLAG = 0x80000000;
+
+    // Bound on spins while waiting for a queue slot that another running
+    // wavefront has claimed but not yet written or cleared.
+    static constexpr uint32_t SLOT_SPIN_LIMIT = 1u << 22;
+    // A wavefront with an empty stash leaves the launch after this many
+    // consecutive idle polls. It holds no promises, so leaving is always
+    // safe; the wavefronts that still hold work finish it, and completion is
+    // decided by the last one out.
+    static constexpr uint32_t IDLE_EXIT_POLLS = 1u << 16;
+
+    struct ThreadContext {
+        unsigned int thread_id;
+        unsigned int rand_state;
+    };
+
+    // Per-wavefront scheduler state, in LDS.
+    struct GroupContext {
+        uint32_t words[LINKS][WORDS][64];
+        uint32_t link_op[LINKS];
+        uint32_t link_count[LINKS];
+        uint32_t open[OPS > 0 ? OPS : 1];
+        uint32_t free_mask;
+        uint32_t ready_mask;
+        uint32_t spare[SPARES];
+        uint32_t spare_count;
+        uint32_t holding;
+        uint32_t can_make_work;
+        uint32_t maker;
+        uint32_t home;
+        unsigned long long int executed;
+        unsigned long long int spills;
+        unsigned long long int fills;
+        unsigned long long int made;
+        unsigned long long int idle;
+        uint32_t idle_streak;
+    };
+
+    struct DeviceContext {
+        typedef ProgramType ParentProgramType;
+        hz::AsyncStatus* status;
+        hz::LinkQueue* queues;      // [queue_count]
+        unsigned int* queue_slots;  // [queue_count][queue_capacity]
+        hz::LinkQueue* free_queue;
+        unsigned int* free_slots;   // [queue_capacity]
+        unsigned int* arena;        // [links][WORDS][64]
+        unsigned int* link_meta;    // [links]: op | count << 16
+        unsigned int links;
+        unsigned int queue_count;
+        unsigned int queue_mask;    // queue_capacity - 1
+        unsigned int coalesce;      // minimum fill before an open link runs while work can still be made
+    };
+
+    // Owns the arena, queues and status of one program instance (reference
+    // program/async.h:302-384). `arena_links` is the number of 64-promise
+    // links of device memory available for spilled work.
+    struct Instance {
+        unsigned int arena_size;
+        unsigned int queue_count;
+        unsigned int queue_capacity;
+        unsigned int coalesce;
+        util::host::DevBuf<hz::AsyncStatus> status;
+        util::host::DevBuf<hz::LinkQueue> queues;
+        util::host::DevBuf<unsigned int> queue_slots;
+        util::host::DevBuf<hz::LinkQueue> free_queue;
+        util::host::DevBuf<unsigned int> free_slots;
+        util::host::DevBuf<unsigned int> arena;
+        util::host::DevBuf<unsigned int> link_meta;
+        DeviceState device_state;
+        hz::AsyncStatus host_status{};
+
+        static unsigned int pow2_at_least(size_t n) {
+            unsigned int p = 64;
+            while (p < n) p <<= 1;
+            return p;
+        }
+
+        static unsigned int default_queue_count() {
+            size_t f = FRAME_SIZE;
+            if (f >= 8) return 8;
+            return f == 0 ? 1u : (unsigned int)f;
+        }
+
+        Instance(size_t arena_links, DeviceState gs, unsigned int queues_wanted = 0, unsigned int coalesce_min = 64)
+            : arena_size((unsigned int)(arena_links < 1 ? 1 : arena_links)),
+              queue_count(queues_wanted ? queues_wanted : default_queue_count()),
+              queue_capacity(pow2_at_least(arena_links < 1 ? 1 : arena_links)),
+              coalesce(coalesce_min > 64 ? 64 : coalesce_min),
+              status((size_t)1),
+              queues((size_t)queue_count),
+              queue_slots((size_t)queue_count * pow2_at_least(arena_links < 1 ? 1 : arena_links)),
+              free_queue((size_t)1),
+              free_slots((size_t)pow2_at_least(arena_links < 1 ? 1 : arena_links)),
+              arena((size_t)(arena_links < 1 ? 1 : arena_links) * WORDS * 64u),
+              link_meta((size_t)(arena_links < 1 ? 1 : arena_links)),
+              device_state(gs) {
+            if (arena_links > 0x7FFFFFFFull) throw std::runtime_error("AsyncProgram::Instance: arena too large");
+            reset(nullptr);
+            util::host::auto_throw(hipDeviceSynchronize());
+        }
+
+        DeviceContext to_context() {
+            DeviceContext c;
+            c.status = status.data();
+            c.queues = queues.data();
+            c.queue_slots = queue_slots.data();
+            c.free_queue = free_queue.data();
+            c.free_slots = free_slots.data();
+            c.arena = arena.data();
+            c.link_meta = link_meta.data();
+            c.links = arena_size;
+            c.queue_count = queue_count;
+            c.queue_mask = queue_capacity - 1u;
+            c.coalesce = coalesce;
+            return c;
+        }
+
+        // Empties every queue and clears the status (graph-capturable: a kernel, no memset).
+        void reset(hipStream_t stream);
+
+        hz::AsyncStatus read_status() {
+            util::host::auto_throw(
+                hipMemcpy(&host_status, status.data(), sizeof(hz::AsyncStatus), hipMemcpyDeviceToHost));
+            return host_status;
+        }
+
+        // True once every promise has been evaluated and make_work is exhausted.
+        bool complete() {
+            read_status();
+            if (host_status.flags & ARENA_EXHAUSTED_FLAG) {
+                throw std::runtime_error(
+                    "AsyncProgram: the link arena was exhausted and promises were lost; "
+                    "construct the Instance with a larger arena");
+            }
+            if (host_status.flags & QUEUE_STALL_FLAG) {
+                throw std::runtime_error("AsyncProgram: a queue slot wait exceeded its bound");
+            }
+            return (host_status.flags & COMPLETION_FLAG) != 0;
+        }
+
+        void clear_flags() {
+            unsigned int zero = 0;
+            util::host::auto_throw(hipMemcpy(&status.data()->flags, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        }
+
+        unsigned long long int promises() { return read_status().promises; }
+    };
+
+    DeviceContext& _dev_ctx;
+    GroupContext& _grp_ctx;
+    ThreadContext& _thd_ctx;
+
+    DeviceState& device;
+    GroupState& group;
+    ThreadState& thread;
+
+    __device__ AsyncProgram(DeviceContext& d_c, GroupContext& g_c, ThreadContext& t_c, DeviceState& d,
+                            GroupState& g, ThreadState& t)
+        : _dev_ctx(d_c), _grp_ctx(g_c), _thd_ctx(t_c), device(d), group(g), thread(t) {}
+
+   protected:
+    // ---------------------------------------------------------------- status
+    __device__ void set_flags(unsigned int bits) { hz::wave::atomic_or(&_dev_ctx.status->flags, bits); }
+
+    __device__ void add_units(int delta) { hz::wave::atomic_add_acqrel(&_dev_ctx.status->units, delta); }
+
+    // ------------------------------------------------------------ link queues
+    // Both functions are called by one lane.
+    __device__ bool queue_push(hz::LinkQueue* q, unsigned int* slots, unsigned int id) {
+        unsigned int t = hz::wave::atomic_add(&q->tail, 1u);
+        unsigned int* slot = slots + (t & _dev_ctx.queue_mask);
+        uint32_t spins = 0;
+        while (hz::wave::atomic_load_acquire(slot) != hz::LINK_NONE) {
+            if (++spins > SLOT_SPIN_LIMIT) {
+                set_flags(QUEUE_STALL_FLAG);
+                return false;
+            }
+            hz::wave::sleep_short();
+        }
+        hz::wave::atomic_store_release(slot, id);
+        hz::wave::atomic_add_acqrel(&q->count, 1);
+        return true;
+    }
+
+    __device__ unsigned int queue_pop(hz::LinkQueue* q, unsigned int* slots) {
+        if (hz::wave::atomic_load(&q->count) <= 0) return hz::LINK_NONE;
+        int c = hz::wave::atomic_add_acqrel(&q->count, -1);
+        if (c <= 0) {
+            hz::wave::atomic_add(&q->count, 1);
+            return hz::LINK_NONE;
+        }
+        unsigned int h = hz::wave::atomic_add(&q->head, 1u);
+        unsigned int* slot = slots + (h & _dev_ctx.queue_mask);
+        unsigned int id;
+        uint32_t spins = 0;
+        while ((id = hz::wave::atomic_load_acquire(slot)) == hz::LINK_NONE) {
+            if (++spins > SLOT_SPIN_LIMIT) {
+                set_flags(QUEUE_STALL_FLAG);
+                return hz::LINK_NONE;
+            }
+            hz::wave::sleep_short();
+        }
+        hz::wave::atomic_store_release(slot, hz::LINK_NONE);
+        return id;
+    }
+
+    __device__ hz::LinkQueue* work_queue(unsigned int q) { return _dev_ctx.queues + q; }
+    __device__ unsigned int* work_slots(unsigned int q) {
+        return _dev_ctx.queue_slots + (size_t)q * (_dev_ctx.queue_mask + 1u);
+    }
+
+    // ------------------------------------------------------------ arena links
+    // Called by one lane. Returns LINK_NONE when the arena is exhausted.
+    __device__ unsigned int arena_alloc_lane() {
+        GroupContext& g = _grp_ctx;
+        if (g.spare_count > 0) {
+            g.spare_count -= 1;
+            return g.spare[g.spare_count];
+        }
+        unsigned int id = queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
+        if (id != hz::LINK_NONE) return id;
+        unsigned int claim = hz::wave::atomic_load(&_dev_ctx.status->claim);
+        if (claim < _dev_ctx.links) {
+            claim = hz::wave::atomic_add(&_dev_ctx.status->claim, 1u);
+            if (claim < _dev_ctx.links) return claim;
+        }
+        // Another group may have just recycled one.
+        return queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
+    }
+
+    __device__ void arena_free_lane(unsigned int id) {
+        GroupContext& g = _grp_ctx;
+        if (g.spare_count < SPARES) {
+            g.spare[g.spare_count] = id;
+            g.spare_count += 1;
+        } else {
+            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, id);
+        }
+    }
+
+    __device__ unsigned int* arena_link(unsigned int id) { return _dev_ctx.arena + (size_t)id * (WORDS * 64u); }
+
+    // -------------------------------------------------------------- the stash
+    __device__ static uint32_t lowest(uint32_t mask) { return (uint32_t)__builtin_ctz(mask); }
+
+    // Moves stash link L to the arena and queues it. Cooperative over the
+    // active lanes (all 64 in the scheduler loop, a subset inside an async
+    // call). Returns false if the arena is exhausted.
+    __device__ bool spill_link(uint32_t L) {
+        GroupContext& g = _grp_ctx;
+        uint64_t act = hz::wave::active();
+        uint32_t n = hz::wave::popc(act);
+        uint32_t r = hz::wave::rank(act);
+        unsigned int id = 0;
+        if (r == 0) id = arena_alloc_lane();
+        id = hz::wave::first(id);
+        if (id == hz::LINK_NONE) {
+            if (r == 0) set_flags(ARENA_EXHAUSTED_FLAG);
+            return false;
+        }
+        unsigned int* dst = arena_link(id);
+        const uint32_t* src = &g.words[L][0][0];
+        if (n == 64) {
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) dst[w * 64u + r] = src[w * 64u + r];
+        } else {
+            for (uint32_t i = r; i < WORDS * 64u; i += n) dst[i] = src[i];
+        }
+        uint32_t op = hz::wave::first(g.link_op[L]);
+        uint32_t cnt = hz::wave::first(g.link_count[L]);
+        hz::wave::fence_release();
+        if (r == 0) {
+            _dev_ctx.link_meta[id] = op | (cnt << 16);
+            hz::wave::fence_release();
+            add_units(1);
+            queue_push(work_queue(g.home), work_slots(g.home), id);
+            g.free_mask |= (1u << L);
+            g.ready_mask &= ~(1u << L);
+            g.spills += 1;
+        }
+        hz::wave::sync();
+        return true;
+    }
+
+    // Spills ready links until at least `want` stash links are free.
+    __device__ void spill_until_free(uint32_t want) {
+        GroupContext& g = _grp_ctx;
+        while (true) {
+            uint32_t fm = hz::wave::first(g.free_mask);
+            if ((uint32_t)__builtin_popcount(fm) >= want) return;
+            uint32_t rm = hz::wave::first(g.ready_mask);
+            if (rm == 0) return;
+            if (!spill_link(lowest(rm))) return;
+        }
+    }
+
+    // Takes a free stash link, spilling a ready one if none is free. With
+    // at least OPS+3 links and at most one open link per operation, a full
+    // stash always has a ready link. Returns LINK_NONE only if the arena is
+    // exhausted (the flag is then set and the caller drops its promises).
+    __device__ uint32_t take_link() {
+        GroupContext& g = _grp_ctx;
+        uint32_t fm = hz::wave::first(g.free_mask);
+        if (fm == 0) {
+            uint32_t rm = hz::wave::first(g.ready_mask);
+            if (rm == 0 || !spill_link(lowest(rm))) return hz::LINK_NONE;
+            fm = hz::wave::first(g.free_mask);
+        }
+        uint32_t L = lowest(fm);
+        if (hz::wave::leader()) g.free_mask = fm & ~(1u << L);
+        hz::wave::sync();
+        return L;
+    }
+
+    // Pulls up to `max_links` queued links into the stash with all 64 lanes.
+    __device__ uint32_t pull_links(uint32_t max_links) {
+        GroupContext& g = _grp_ctx;
+        const uint32_t lane = hz::wave::lane();
+        uint32_t pulled = 0;
+        for (uint32_t k = 0; k < max_links; ++k) {
+            uint32_t fm = hz::wave::first(g.free_mask);
+            if (fm == 0) break;
+            unsigned int id = hz::LINK_NONE;
+            if (lane == 0) {
+                uint32_t qn = _dev_ctx.queue_count;
+                for (uint32_t i = 0; i < qn && id == hz::LINK_NONE; ++i) {
+                    uint32_t q = g.home + i;
+                    if (q >= qn) q -= qn;
+                    id = queue_pop(work_queue(q), work_slots(q));
+                }
+            }
+            id = hz::wave::first(id);
+            if (id == hz::LINK_NONE) break;
+            hz::wave::fence_acquire();
+            uint32_t L = lowest(fm);
+            const unsigned int* src = arena_link(id);
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][lane] = src[w * 64u + lane];
+            uint32_t meta = hz::wave::first(_dev_ctx.link_meta[id]);
+            if (lane == 0) {
+                g.link_op[L] = meta & 0xFFFFu;
+                g.link_count[L] = meta >> 16;
+                g.free_mask = fm & ~(1u << L);
+                g.ready_mask |= (1u << L);
+                g.fills += 1;
+                arena_free_lane(id);
+                if (g.holding) {
+                    add_units(-1);
+                } else {
+                    g.holding = 1;
+                }
+            }
+            hz::wave::sync();
+            pulled++;
+        }
+        return pulled;
+    }
+
+    // Appends one promise per active lane to the open link of operation D.
+    __device__ void stash_append(uint32_t D, const uint32_t* words) {
+        GroupContext& g = _grp_ctx;
+        uint64_t act = hz::wave::active();
+        uint32_t n = hz::wave::popc(act);
+        uint32_t r = hz::wave::rank(act);
+        uint32_t L = hz::wave::first(g.open[D]);
+        uint32_t c = 0;
+        if (L == hz::LINK_NONE) {
+            L = take_link();
+            if (L == hz::LINK_NONE) return;
+            if (r == 0) {
+                g.link_op[L] = D;
+                g.link_count[L] = 0;
+                g.open[D] = L;
+            }
+            hz::wave::sync();
+        } else {
+            c = hz::wave::first(g.link_count[L]);
+        }
+        uint32_t room = 64u - c;
+        if (r < room) {
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
+        }
+        if (n < room) {
+            if (r == 0) g.link_count[L] = c + n;
+            hz::wave::sync();
+            return;
+        }
+        // L is now full: close it, then open another for the overflow.
+        if (r == 0) {
+            g.link_count[L] = 64u;
+            g.ready_mask |= (1u << L);
+            g.open[D] = hz::LINK_NONE;
+        }
+        hz::wave::sync();
+        uint32_t rest = n - room;
+        if (rest == 0) return;
+        uint32_t L2 = take_link();
+        if (L2 == hz::LINK_NONE) return;
+        if (r >= room) {
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L2][w][r - room] = words[w];
+        }
+        if (r == 0) {
+            g.link_op[L2] = D;
+            g.link_count[L2] = rest;
+            g.open[D] = L2;
+        }
+        hz::wave::sync();
+    }
+
+    template <typename TYPE>
+    __device__ void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
+        static_assert(Lookup<TYPE>::type::CONTAINED, "operation is not part of this program's OpSet");
+        constexpr OpDisc D = Lookup<TYPE>::type::DISC;
+        PromiseUnionType pu;
+        pu.template cast<TYPE>() = param_value;
+        uint32_t words[WORDS];
+        __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
+        stash_append(D, words);
+    }
+
+    template <typename TYPE>
+    __device__ void immediate_call_cast(Promise<TYPE> param_value) {
+        param_value(*this);
+    }
+
+    // Stash link with the most promises among the open ones, or LINK_NONE.
+    __device__ uint32_t fullest_open(uint32_t& count) {
+        GroupContext& g = _grp_ctx;
+        uint32_t best = hz::LINK_NONE, best_count = 0;
+        for (uint32_t d = 0; d < OPS; ++d) {
+            uint32_t L = hz::wave::first(g.open[d]);
+            if (L == hz::LINK_NONE) continue;
+            uint32_t c = hz::wave::first(g.link_count[L]);
+            if (c > best_count) {
+                best = L;
+                best_count = c;
+            }
+        }
+        count = best_count;
+        return best;
+    }
+
+    __device__ bool stash_empty() { return hz::wave::first(_grp_ctx.free_mask) == ALL_LINKS; }
+
+    __device__ void drop_holding_if_empty() {
+        GroupContext& g = _grp_ctx;
+        if (stash_empty() && hz::wave::first(g.holding)) {
+            if (hz::wave::lane() == 0) {
+                g.holding = 0;
+                add_units(-1);
+            }
+            hz::wave::sync();
+        }
+    }
+
+    __device__ void make_work_step() {
+        GroupContext& g = _grp_ctx;
+        const uint32_t lane = hz::wave::lane();
+        if (lane == 0) {
+            if (!g.holding) {
+                g.holding = 1;
+                add_units(1);
+            }
+            g.made += 1;
+        }
+        hz::wave::sync();
+        bool mine = PROGRAM_SPEC::make_work(*this);
+        bool more = hz::wave::ballot(mine) != 0;
+        if (lane == 0) {
+            g.can_make_work = more ? 1u : 0u;
+            if (!more && g.maker) {
+                g.maker = 0;
+                hz::wave::atomic_add_acqrel(&_dev_ctx.status->makers, -1);
+            }
+        }
+        hz::wave::sync();
+        drop_holding_if_empty();
+    }
+
+    __device__ void run_link(uint32_t L) {
+        GroupContext& g = _grp_ctx;
+        const uint32_t lane = hz::wave::lane();
+        uint32_t op = hz::wave::first(g.link_op[L]);
+        uint32_t cnt = hz::wave::first(g.link_count[L]);
+        uint32_t words[WORDS];
+        if (lane < cnt) {
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+        }
+        hz::wave::sync();
+        if (lane == 0) {
+            g.free_mask |= (1u << L);
+            g.ready_mask &= ~(1u << L);
+            g.executed += cnt;
+        }
+        hz::wave::sync();
+        if (lane < cnt) {
+            PromiseUnionType pu;
+            __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
+            pu.template loose_eval<ProgramType>(*this, op);
+        }
+        hz::wave::sync();
+        drop_holding_if_empty();
+    }
+
+    __device__ bool queues_hint() {
+        hz::LinkQueue* q = work_queue(hz::wave::first(_grp_ctx.home));
+        return hz::wave::first((int32_t)hz::wave::atomic_load(&q->count)) > 0;
+    }
+
+    enum CycleResult { RUN = 0, DONE = 1 };
+
+    // One scheduling cycle (reference exec_cycle, program/async.h:2155-2319).
+    __device__ CycleResult exec_cycle() {
+        GroupContext& g = _grp_ctx;
+        spill_until_free(MARGIN);
+
+        uint32_t rm = hz::wave::first(g.ready_mask);
+        if (rm == 0) {
+            if (queues_hint()) {
+                pull_links(1);
+                rm = hz::wave::first(g.ready_mask);
+            }
+            // Make work while no link is full, bounded so open links still run.
+            for (uint32_t tries = 0; rm == 0 && tries < 4 && hz::wave::first(g.can_make_work); ++tries) {
+                uint32_t c = 0;
+                fullest_open(c);
+                if (tries > 0 && c >= _dev_ctx.coalesce) break;
+                make_work_step();
+                rm = hz::wave::first(g.ready_mask);
+            }
+        }
+
+        uint32_t L = hz::LINK_NONE;
+        if (rm != 0) {
+            L = lowest(rm);
+        } else {
+            uint32_t c = 0;
+            L = fullest_open(c);
+            if (L != hz::LINK_NONE) {
+                uint32_t op = hz::wave::first(g.link_op[L]);
+                if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
+                hz::wave::sync();
+            }
+        }
+
+        if (L != hz::LINK_NONE) {
+            if (hz::wave::lane() == 0) g.idle_streak = 0;
+            run_link(L);
+            return RUN;
+        }
+
+        // Nothing local: look for queued links anywhere, then test for quiescence.
+        if (pull_links(1) > 0) return RUN;
+        if (hz::wave::first(g.can_make_work)) return RUN;
+        uint32_t streak = hz::wave::first(g.idle_streak);
+        if (streak >= IDLE_EXIT_POLLS) return DONE;
+        int makers = 0, units = 0;
+        unsigned int flags = 0;
+        if (hz::wave::lane() == 0) {
+            makers = hz::wave::atomic_load_acquire(&_dev_ctx.status->makers);
+            units = hz::wave::atomic_load_acquire(&_dev_ctx.status->units);
+            flags = hz::wave::atomic_load(&_dev_ctx.status->flags);
+            g.idle += 1;
+            g.idle_streak = streak + 1;
+        }
+        makers = hz::wave::first(makers);
+        units = hz::wave::first(units);
+        flags = hz::wave::first(flags);
+        if ((makers == 0 && units == 0) || (flags & ~COMPLETION_FLAG) != 0) return DONE;
+        hz::wave::sleep_short();
+        return RUN;
+    }
+
+    __device__ void init_group() {
+        GroupContext& g = _grp_ctx;
+        if (hz::wave::lane() == 0) {
+            g.free_mask = ALL_LINKS;
+            g.ready_mask = 0;
+            for (uint32_t d = 0; d < OPS; ++d) g.open[d] = hz::LINK_NONE;
+            g.spare_count = 0;
+            g.holding = 0;
+            g.can_make_work = 1;
+            g.maker = 1;
+            g.home = blockIdx.x % _dev_ctx.queue_count;
+            g.executed = g.spills = g.fills = g.made = g.idle = 0;
+            g.idle_streak = 0;
+            hz::wave::atomic_add_acqrel(&_dev_ctx.status->makers, 1);
+        }
+        hz::wave::sync();
+    }
+
+    __device__ void init_thread() {
+        _thd_ctx.thread_id = blockIdx.x * 64u + hz::wave::lane();
+        _thd_ctx.rand_state = _thd_ctx.thread_id;
+    }
+
+    // Leaves nothing in LDS: every stash link goes back to the arena
+    // (reference cleanup_runtime, program/async.h:2323-2393).
+    __device__ void cleanup_runtime() {
+        GroupContext& g = _grp_ctx;
+        const uint32_t lane = hz::wave::lane();
+        if (lane == 0) {
+            for (uint32_t d = 0; d < OPS; ++d) g.open[d] = hz::LINK_NONE;
+        }
+        hz::wave::sync();
+        while (true) {
+            uint32_t used = ALL_LINKS & ~hz::wave::first(g.free_mask);
+            if (used == 0) break;
+            if (!spill_link(lowest(used))) {
+                // Arena exhausted: the flag is set and the promises are lost.
+                if (lane == 0) g.free_mask = ALL_LINKS;
+                hz::wave::sync();
+                break;
+            }
+        }
+        if (lane == 0) {
+            if (g.holding) {
+                g.holding = 0;
+                add_units(-1);
+            }
+            while (g.spare_count > 0) {
+                g.spare_count -= 1;
+                queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, g.spare[g.spare_count]);
+            }
+            if (g.maker) {
+                if (g.can_make_work) set_flags(EARLY_HALT_FLAG);
+                g.maker = 0;
+                hz::wave::atomic_add_acqrel(&_dev_ctx.status->makers, -1);
+            }
+            hz::AsyncStatus* st = _dev_ctx.status;
+            hz::wave::atomic_add(&st->promises, g.executed);
+            if (g.spills) hz::wave::atomic_add(&st->spills, g.spills);
+            if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
+            if (g.made) hz::wave::atomic_add(&st->make_work_calls, g.made);
+            if (g.idle) hz::wave::atomic_add(&st->idle_polls, g.idle);
+            unsigned int order = hz::wave::atomic_add_acqrel(&st->checkout, 1u);
+            if (order == gridDim.x - 1) {
+                unsigned int old = hz::wave::atomic_and(&st->flags, ~EARLY_HALT_FLAG);
+                int units = hz::wave::atomic_load_acquire(&st->units);
+                if ((old & EARLY_HALT_FLAG) == 0 && units == 0) set_flags(COMPLETION_FLAG);
+                hz::wave::atomic_add(&st->launches, 1ull);
+                hz::wave::atomic_store(&st->checkout, 0u);
+            }
+        }
+        hz::wave::sync();
+    }
+
+   public:
+    __device__ void init_program() {}
+
+    // The persistent loop (reference program/async.h:2625-2668).
+    __device__ void exec(unsigned int cycle_count) {
+        init_group();
+        init_thread();
+        PROGRAM_SPEC::initialize(*this);
+        for (unsigned int cycle = 0; cycle < cycle_count; ++cycle) {
+            if ((cycle & 63u) == 63u) {
+                unsigned int flags = 0;
+                if (hz::wave::lane() == 0) flags = hz::wave::atomic_load(&_dev_ctx.status->flags);
+                flags = hz::wave::first(flags);
+                if (flags & ~COMPLETION_FLAG) break;
+            }
+            if (exec_cycle() == DONE) break;
+        }
+        PROGRAM_SPEC::finalize(*this);
+        cleanup_runtime();
+    }
+
+    template <typename TYPE, typename... ARGS>
+    __device__ void async(ARGS... args) {
+        async_call_cast<TYPE>(0, Promise<TYPE>(args...));
+    }
+
+    template <typename TYPE>
+    __device__ void async_call(Promise<TYPE> promise) {
+        async_call_cast<TYPE>(0, promise);
+    }
+
+    template <typename TYPE, typename... ARGS>
+    __device__ void sync(ARGS... args) {
+        immediate_call_cast<TYPE>(Promise<TYPE>(args...));
+    }
+
+    template <typename TYPE>
+    __device__ void sync_call(Promise<TYPE> promise) {
+        immediate_call_cast<TYPE>(promise);
+    }
+
+    // The async program has no queue fill fraction (reference async.h:2900-2904).
+    template <typename TYPE>
+    __device__ float load_fraction() {
+        return NAN;
+    }
+
+    __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
+
+    __device__ static void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int cycles) {
+        __shared__ GroupContext gctx;
+        __shared__ GroupState group_state;
+        ThreadContext tctx;
+        ThreadState thread_state;
+        ProgramType prog(ctx, gctx, tctx, state, group_state, thread_state);
+        prog.exec(cycles);
+    }
+
+    // Empties the queues and status of an instance; grid-stride over the slot arrays.
+    __device__ static void kernel_reset(DeviceContext ctx) {
+        size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        size_t stride = (size_t)gridDim.x * blockDim.x;
+        size_t cap = (size_t)ctx.queue_mask + 1u;
+        size_t nslots = cap * ctx.queue_count;
+        for (size_t i = tid; i < nslots; i += stride) ctx.queue_slots[i] = hz::LINK_NONE;
+        for (size_t i = tid; i < cap; i += stride) ctx.free_slots[i] = hz::LINK_NONE;
+        for (size_t i = tid; i < ctx.queue_count; i += stride) {
+            ctx.queues[i].tail = 0;
+            ctx.queues[i].head = 0;
+            ctx.queues[i].count = 0;
+        }
+        if (tid == 0) {
+            ctx.free_queue->tail = 0;
+            ctx.free_queue->head = 0;
+            ctx.free_queue->count = 0;
+            hz::AsyncStatus* st = ctx.status;
+            st->flags = 0;
+            st->units = 0;
+            st->makers = 0;
+            st->checkout = 0;
+            st->claim = 0;
+            st->promises = st->spills = st->fills = st->make_work_calls = st->launches = st->idle_polls = 0;
+        }
+    }
+};

@@ -1,0 +1,371 @@
+// Operation sets, promises and promise unions: the type-level ABI that program
+// specifications are written against.
+//
+// Same names and meaning as the reference's async/promise.h (OpUnion :17,
+// OpUnionLookup :65-91, ArgTuple :247-305, OpType :312-319, Promise :439-490,
+// PromiseUnion :503-696, PromiseEnum :705-742), so specifications written for
+// the reference compile unchanged. The storage is laid out differently:
+//  - a Promise of a void operation carries only its arguments (the reference
+//    adds an unused return address), so the Collatz promise is 16 bytes, not 32;
+//  - ArgTuple has no trailing empty member, so argument packs are packed the
+//    way the platform ABI packs a plain struct of the same fields.
+// Packed promises are copied word by word into structure-of-arrays work links
+// (one 32-bit row per word, one column per lane of a wavefront).
+#pragma once
+
+#include "portable.h"
+
+using OpDisc = unsigned int;
+
+template <typename... OPS>
+struct OpUnion {};
+
+template <typename OPERATION>
+struct Promise;
+
+template <typename OP_UNION>
+union PromiseUnion;
+
+template <typename OP_UNION>
+struct PromiseEnum;
+
+struct VoidState {};
+
+template <typename OP_SET, typename ADR_TYPE = unsigned int>
+struct RemappingBarrier;
+
+template <typename OP_SET, typename ADR_TYPE = unsigned int>
+struct UnitBarrier;
+
+template <typename TYPE, typename BARRIER>
+struct Future;
+
+namespace hz {
+namespace detail {
+
+template <typename Q, typename... OPS>
+struct IndexOf;
+
+template <typename Q>
+struct IndexOf<Q> {
+    static constexpr bool found = false;
+    static constexpr unsigned value = 0;
+};
+
+template <typename Q, typename H, typename... T>
+struct IndexOf<Q, H, T...> {
+    static constexpr bool found = std::is_same<Q, H>::value || IndexOf<Q, T...>::found;
+    static constexpr unsigned value =
+        std::is_same<Q, H>::value ? 0u : 1u + IndexOf<Q, T...>::value;
+};
+
+template <unsigned I, typename... OPS>
+struct OpAt;
+
+template <typename H, typename... T>
+struct OpAt<0, H, T...> {
+    using type = H;
+};
+
+template <unsigned I, typename H, typename... T>
+struct OpAt<I, H, T...> {
+    using type = typename OpAt<I - 1, T...>::type;
+};
+
+}  // namespace detail
+}  // namespace hz
+
+// Compile-time facts about one operation type relative to an operation set.
+// DISC is the operation's position in the set (first operation = 0), which is
+// the discriminant stored with every work link.
+template <typename QUERY, typename OP_UNION>
+struct OpUnionLookup;
+
+template <typename QUERY, typename... OPS>
+struct OpUnionLookup<QUERY, OpUnion<OPS...>> {
+    static constexpr bool CONTAINED = hz::detail::IndexOf<QUERY, OPS...>::found;
+    static constexpr OpDisc DEPTH = sizeof...(OPS);
+    static constexpr OpDisc DISC = hz::detail::IndexOf<QUERY, OPS...>::value;
+};
+
+template <typename HEAD, typename TAIL_UNION>
+struct OpUnionAppend;
+
+template <typename HEAD, typename... TAIL>
+struct OpUnionAppend<HEAD, OpUnion<TAIL...>> {
+    using Type = OpUnion<HEAD, TAIL...>;
+};
+
+// Subset / equality relations between two operation sets.
+template <typename LEFT, typename RIGHT>
+struct OpUnionPair;
+
+template <typename... L, typename... R>
+struct OpUnionPair<OpUnion<L...>, OpUnion<R...>> {
+    static constexpr bool LEFT_SUBSET = (true && ... && OpUnionLookup<L, OpUnion<R...>>::CONTAINED);
+    static constexpr bool RIGHT_SUBSET = (true && ... && OpUnionLookup<R, OpUnion<L...>>::CONTAINED);
+    static constexpr bool EQUAL = LEFT_SUBSET && RIGHT_SUBSET;
+};
+
+// A semaphore that also records what kind of object owns it.
+struct TaggedSemaphore {
+    unsigned int sem;
+    unsigned int tag;
+
+    TaggedSemaphore() = default;
+    HZ_HD TaggedSemaphore(unsigned int sem_val, unsigned int tag_val) : sem(sem_val), tag(tag_val) {}
+};
+
+// Marks an argument that the reference loads lazily (async/promise.h:204-240).
+// Here the value is read eagerly when the promise is built.
+template <typename TYPE>
+struct LazyLoad {
+    using Type = TYPE;
+    Type& reference;
+    HZ_HD LazyLoad(Type& ref) : reference(ref) {}
+    HZ_HD operator TYPE() { return reference; }
+};
+
+template <typename... TYPES>
+HZ_HD bool has_load(TYPES...) { return false; }
+template <typename HEAD, typename... TAIL>
+HZ_HD bool has_load(LazyLoad<HEAD>, TAIL...) { return true; }
+template <typename HEAD, typename... TAIL>
+HZ_HD bool has_load(HEAD, TAIL... tail) { return has_load(tail...); }
+
+// Packed argument storage. A one-element tuple has no tail member so that a
+// pack of (u64, u32, u32) occupies exactly 16 bytes.
+template <typename... TAIL>
+struct ArgTuple;
+
+template <>
+struct ArgTuple<> {
+    ArgTuple() = default;
+};
+
+template <typename HEAD>
+struct ArgTuple<HEAD> {
+    using Head = HEAD;
+    Head head;
+    ArgTuple() = default;
+    HZ_HD ArgTuple(HEAD h) : head(h) {}
+};
+
+template <typename HEAD, typename NEXT, typename... TAIL>
+struct ArgTuple<HEAD, NEXT, TAIL...> {
+    using Head = HEAD;
+    using Tail = ArgTuple<NEXT, TAIL...>;
+    Head head;
+    Tail tail;
+    ArgTuple() = default;
+    HZ_HD ArgTuple(HEAD h, NEXT n, TAIL... t) : head(h), tail(n, t...) {}
+};
+
+// Deduces the return and argument types of an operation from `using Type = R(*)(Args...)`.
+template <typename TYPE>
+struct OpType;
+
+template <typename RETURN, typename... ARGS>
+struct OpType<RETURN (*)(ARGS...)> {
+    typedef RETURN Return;
+    typedef ArgTuple<ARGS...> Args;
+};
+
+// Where a value-returning operation delivers its result: the semaphore of the
+// future/barrier to decrement and the value slot to fill.
+template <typename TYPE>
+struct ReturnAdr {
+    using Type = TYPE;
+    TaggedSemaphore* base;
+    Type* dest;
+};
+
+template <>
+struct ReturnAdr<void> {
+    using Type = void;
+    TaggedSemaphore* base;
+};
+
+namespace hz {
+namespace detail {
+
+// Void operations carry no return address at all (empty base).
+template <typename RETURN>
+struct PromiseReturn {
+    ReturnAdr<RETURN> ret;
+};
+template <>
+struct PromiseReturn<void> {};
+
+template <typename OP, typename PROGRAM, typename TUPLE, typename... UNROLLED>
+struct Unpack;
+
+template <typename OP, typename PROGRAM, typename... UNROLLED>
+struct Unpack<OP, PROGRAM, ArgTuple<>, UNROLLED...> {
+    template <typename R>
+    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<>&, UNROLLED... u) {
+        return OP::template eval<PROGRAM>(prog, u...);
+    }
+};
+
+template <typename OP, typename PROGRAM, typename H, typename... UNROLLED>
+struct Unpack<OP, PROGRAM, ArgTuple<H>, UNROLLED...> {
+    template <typename R>
+    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<H>& t, UNROLLED... u) {
+        return OP::template eval<PROGRAM>(prog, u..., t.head);
+    }
+};
+
+template <typename OP, typename PROGRAM, typename H, typename N, typename... T, typename... UNROLLED>
+struct Unpack<OP, PROGRAM, ArgTuple<H, N, T...>, UNROLLED...> {
+    template <typename R>
+    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<H, N, T...>& t, UNROLLED... u) {
+        return Unpack<OP, PROGRAM, ArgTuple<N, T...>, UNROLLED..., H>::template call<R>(prog, t.tail, u..., t.head);
+    }
+};
+
+}  // namespace detail
+}  // namespace hz
+
+template <typename OPERATION>
+struct Promise : hz::detail::PromiseReturn<typename OpType<typename OPERATION::Type>::Return> {
+    typedef typename OpType<typename OPERATION::Type>::Return Return;
+    typedef typename OpType<typename OPERATION::Type>::Args Args;
+
+    Args args;
+
+    Promise() = default;
+
+    template <typename... ARGS>
+    HZ_HD Promise(ARGS... a) : args(a...) {}
+
+    // Evaluates the operation with the stored arguments on the given program.
+    template <typename PROGRAM>
+    __device__ __host__ Return operator()(PROGRAM program) {
+        return hz::detail::Unpack<OPERATION, PROGRAM, Args>::template call<Return>(program, args);
+    }
+};
+
+// A union over the promise types of an operation set. Which member is live is
+// not stored here; work links carry it as their discriminant.
+template <>
+union PromiseUnion<OpUnion<>> {
+    struct Info {
+        static constexpr OpDisc COUNT = 0;
+    };
+    template <typename TYPE>
+    struct Lookup {
+        typedef OpUnionLookup<TYPE, OpUnion<>> type;
+    };
+    template <typename PROGRAM, typename TYPE>
+    __device__ __host__ void rigid_eval(PROGRAM) {}
+    template <typename PROGRAM>
+    __device__ __host__ void loose_eval(PROGRAM, OpDisc) {}
+    HZ_HD void dyn_copy_as(OpDisc, PromiseUnion<OpUnion<>>&) {}
+    PromiseUnion() = default;
+};
+
+template <typename HEAD, typename... TAIL>
+union PromiseUnion<OpUnion<HEAD, TAIL...>> {
+    using Head = Promise<HEAD>;
+    using Tail = PromiseUnion<OpUnion<TAIL...>>;
+    using OpSet = OpUnion<HEAD, TAIL...>;
+    using TailOpSet = OpUnion<TAIL...>;
+
+    Head head_form;
+    Tail tail_form;
+
+    struct Info {
+        static constexpr OpDisc COUNT = sizeof...(TAIL) + 1;
+        static constexpr OpDisc INDEX = 0;
+    };
+
+    template <typename TYPE>
+    struct Lookup {
+        typedef OpUnionLookup<TYPE, OpSet> type;
+    };
+
+    PromiseUnion() = default;
+
+    template <typename OP_TYPE>
+    HZ_HD PromiseUnion(Promise<OP_TYPE> prom) {
+        static_assert(OpUnionLookup<OP_TYPE, OpSet>::CONTAINED,
+                      "Promise type is not part of this promise union's operation set.");
+        cast<OP_TYPE>() = prom;
+    }
+
+    // Views the union as the promise type of TYPE.
+    template <typename TYPE>
+    HZ_HD Promise<TYPE>& cast() {
+        static_assert(OpUnionLookup<TYPE, OpSet>::CONTAINED,
+                      "Promise type is not part of this promise union's operation set.");
+        if constexpr (std::is_same<TYPE, HEAD>::value) {
+            return head_form;
+        } else {
+            return tail_form.template cast<TYPE>();
+        }
+    }
+
+    // Evaluates the union as a promise of the statically known operation TYPE.
+    template <typename PROGRAM, typename TYPE>
+    __device__ __host__ void rigid_eval(PROGRAM program) {
+        cast<TYPE>()(program);
+    }
+
+    // Evaluates the union as a promise of the operation with discriminant `disc`.
+    template <typename PROGRAM>
+    __device__ __host__ void loose_eval(PROGRAM program, OpDisc disc) {
+        if (disc == 0) {
+            head_form(program);
+        } else {
+            if constexpr (sizeof...(TAIL) > 0) {
+                tail_form.template loose_eval<PROGRAM>(program, disc - 1);
+            }
+        }
+    }
+
+    template <typename TYPE>
+    HZ_HD void copy_as(PromiseUnion<OpSet>& other) {
+        cast<TYPE>() = other.template cast<TYPE>();
+    }
+
+    HZ_HD void dyn_copy_as(OpDisc disc, PromiseUnion<OpSet>& other) {
+        if (disc == 0) {
+            head_form = other.head_form;
+        } else {
+            tail_form.dyn_copy_as(disc - 1, other.tail_form);
+        }
+    }
+};
+
+// A promise union tagged with its discriminant.
+template <typename OP_SET>
+struct PromiseEnum {
+    using UnionType = PromiseUnion<OP_SET>;
+    UnionType data;
+    OpDisc disc;
+
+    PromiseEnum() = default;
+    HZ_HD PromiseEnum(UnionType uni, OpDisc d) : data(uni), disc(d) {}
+
+    template <typename OP_TYPE>
+    HZ_HD PromiseEnum(Promise<OP_TYPE> promise)
+        : data(promise), disc(UnionType::template Lookup<OP_TYPE>::type::DISC) {}
+};
+
+namespace hz {
+
+// Number of 32-bit rows one promise of OP_SET occupies in a work link.
+template <typename OP_SET>
+struct PromiseWords {
+    static constexpr uint32_t value = WordCount<PromiseUnion<OP_SET>>::value;
+};
+
+template <typename OP_SET>
+struct OpCount;
+template <typename... OPS>
+struct OpCount<OpUnion<OPS...>> {
+    static constexpr uint32_t value = sizeof...(OPS);
+};
+
+}  // namespace hz

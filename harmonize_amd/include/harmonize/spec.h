@@ -1,0 +1,52 @@
+// Reads a program specification's optional members, falling back to defaults
+// when a member is absent. Same contract as the reference's MEMBER_SWITCH /
+// CONST_SWITCH (program/async.h:26-36, 101-131; program/event.h:22-32,
+// 59-75): OpSet, DeviceState, GroupState, ThreadState, AdrType and the
+// *_SIZE constants are all optional. A state declared as a member template
+// (as the reference's barrier demo does) is not a type and falls back to
+// VoidState, as in the reference.
+#pragma once
+
+#include "promise.h"
+
+namespace hz {
+namespace spec {
+
+#define HZ_TYPE_OR(NAME)                                                     \
+    template <typename S, typename D, typename = void>                       \
+    struct NAME##Or {                                                        \
+        using type = D;                                                      \
+    };                                                                       \
+    template <typename S, typename D>                                        \
+    struct NAME##Or<S, D, std::void_t<typename S::NAME>> {                   \
+        using type = typename S::NAME;                                       \
+    };
+
+HZ_TYPE_OR(OpSet)
+HZ_TYPE_OR(DeviceState)
+HZ_TYPE_OR(GroupState)
+HZ_TYPE_OR(ThreadState)
+HZ_TYPE_OR(AdrType)
+#undef HZ_TYPE_OR
+
+#define HZ_CONST_OR(NAME)                                                    \
+    template <typename S, size_t D, typename = void>                         \
+    struct NAME##Or {                                                        \
+        static constexpr size_t value = D;                                   \
+        static constexpr bool given = false;                                 \
+    };                                                                       \
+    template <typename S, size_t D>                                          \
+    struct NAME##Or<S, D, std::void_t<decltype(S::NAME)>> {                  \
+        static constexpr size_t value = (size_t)S::NAME;                     \
+        static constexpr bool given = true;                                  \
+    };
+
+HZ_CONST_OR(STASH_SIZE)
+HZ_CONST_OR(FRAME_SIZE)
+HZ_CONST_OR(POOL_SIZE)
+HZ_CONST_OR(STACK_SIZE)
+HZ_CONST_OR(GROUP_SIZE)
+#undef HZ_CONST_OR
+
+}  // namespace spec
+}  // namespace hz

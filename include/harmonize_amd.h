@@ -1,0 +1,166 @@
+/*
+ * C-ABI boundary of libharmonize_amd.so, the gfx950 asynchronous execution
+ * runtime. Plain C: opaque handles, plain pointers and sizes, no exceptions
+ * across the boundary. Device pointers are HIP device allocations on the
+ * current device (harmonize_alloc_device_bytes or any hipMalloc).
+ *
+ * Two layers:
+ *  1. The per-program entry points that the reference generates for its
+ *     Python binding (harmonize/python/templates.py), with the same names,
+ *     argument order and meaning, for every built-in specification
+ *     <spec>_<kind> with <spec> in {collatz, photon, bfs} and <kind> in
+ *     {event, async}. These keep the reference's `void` returns; failures
+ *     are reported through hz_last_error() instead of C++ exceptions
+ *     crossing the boundary as in the reference.
+ *  2. hz_* workload drivers that run a whole program to completion and
+ *     report statistics. They return 0 on success and a negative status on
+ *     failure (message in hz_last_error()).
+ */
+#ifndef HARMONIZE_AMD_H
+#define HARMONIZE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+
+#define HZ_OK 0
+#define HZ_ERR_ARGUMENT (-1)
+#define HZ_ERR_RUNTIME (-2)   /* HIP error, arena/queue exhaustion, overflow */
+#define HZ_ERR_UNSUPPORTED (-3)
+
+/* Message of the last failure on this thread, or NULL if the last call succeeded. */
+const char* hz_last_error(void);
+
+/* Program kinds. */
+#define HZ_EVENT 0 /* EventProgram: one launch per generation */
+#define HZ_ASYNC 1 /* AsyncProgram: persistent coalescing scheduler */
+#define HZ_PLAIN 2 /* one thread per work item, no runtime (Collatz only) */
+
+/* --------------------------------------- device memory helpers (layer 1)
+ * Replace templates.py:307-367 (harmonize_alloc_device_bytes,
+ * harmonize_alloc_managed_bytes, harmonize_free_device_bytes,
+ * harmonize_memcpy_host_to_device, harmonize_memcpy_device_to_host; the
+ * reference's device-to-host copy passes the host-to-device kind,
+ * templates.py:365, which is not repeated here). NULL on failure. */
+void* harmonize_alloc_device_bytes(size_t size);
+void* harmonize_alloc_managed_bytes(size_t size);
+void harmonize_free_device_bytes(void* ptr);
+void harmonize_memcpy_host_to_device(void* dev_ptr, void* host_ptr, size_t byte_count);
+void harmonize_memcpy_device_to_host(void* host_ptr, void* dev_ptr, size_t byte_count);
+
+/* ------------------------------------------ device states of the built-ins */
+
+/* Collatz step counts. make_work claims indices of `starts` (or uses the
+ * index itself as the start when starts is NULL) from `iterator`, a device
+ * pointer to a {uint32 value, uint32 limit} pair. steps[i] receives the
+ * number of Collatz steps of start i, or HZ_COLLATZ_OVERFLOW if the
+ * trajectory leaves 64 bits. */
+#define HZ_COLLATZ_OVERFLOW 0xFFFFFFFFu
+typedef struct hz_collatz_state {
+    const uint64_t* starts;
+    uint32_t* steps;
+    void* iterator;
+} hz_collatz_state;
+
+/* ------------------------------------------- per-program entry points
+ * For each <suffix> in collatz_event, collatz_async:
+ *
+ *   void* alloc_state_<suffix>(void);
+ *       Device allocation sized for the program's device state
+ *       (templates.py:73-83).
+ *   void free_state_<suffix>(void* state);                    (:154-159)
+ *   void load_state_device_<suffix>(void* host, void* dev);   (:85-104)
+ *   void store_state_device_<suffix>(void* dev, void* host);  (:106-125)
+ *       Copy the whole device state between host and device.
+ *   void* alloc_program_<suffix>(void* dev_state, size_t size);
+ *       New instance bound to a device state from alloc_state_<suffix>.
+ *       size = io capacity per operation (event, :43-52) or arena links of
+ *       64 promises (async, :54-63). NULL on failure.
+ *   void free_program_<suffix>(void* instance);               (:65-71)
+ *   void init_program_<suffix>(void* instance, size_t grid);  (:3-14)
+ *   void exec_program_<suffix>(void* instance, size_t grid, size_t cycles);
+ *       One launch of `grid` groups, then a device synchronize (:17-40).
+ *       The device state is re-read from its device copy at every launch,
+ *       so stores between launches take effect as in the reference.
+ *   int complete_<suffix>(void* instance);                    (:161-168)
+ *       1 when finished, 0 when not, -1 on error.
+ *   void clear_flags_<suffix>(void* instance);                (:170-177)
+ *   void set_device_<suffix>(int device);                     (:179-184)
+ */
+#define HZ_DECLARE_PROGRAM_ABI(suffix)                                              \
+    void* alloc_state_##suffix(void);                                               \
+    void free_state_##suffix(void* state);                                          \
+    void load_state_device_##suffix(void* host_ptr, void* dev_ptr);                 \
+    void store_state_device_##suffix(void* dev_ptr, void* host_ptr);                \
+    void* alloc_program_##suffix(void* device_arg, size_t size);                    \
+    void free_program_##suffix(void* instance_ptr);                                 \
+    void init_program_##suffix(void* instance_ptr, size_t grid_size);               \
+    void exec_program_##suffix(void* instance_ptr, size_t grid_size, size_t cycle_count); \
+    int complete_##suffix(void* instance_ptr);                                      \
+    void clear_flags_##suffix(void* instance_ptr);                                  \
+    void set_device_##suffix(int device);
+
+HZ_DECLARE_PROGRAM_ABI(collatz_event)
+HZ_DECLARE_PROGRAM_ABI(collatz_async)
+
+/* ------------------------------------------------------ drivers (layer 2) */
+
+typedef struct hz_run_config {
+    uint32_t groups;   /* wavefront groups per launch; 0 = fill the device */
+    uint32_t cycles;   /* async: cycle budget per launch (0 = 2^24);
+                          event: chunk size in wavefronts of promises (0 = 4) */
+    uint64_t arena;    /* async: arena links of 64 promises; event: queue
+                          capacity per operation, in promises (0 = default) */
+    uint32_t queues;   /* async: number of link queues (0 = 8) */
+    uint32_t coalesce; /* async: minimum promises in an open link before it
+                          runs while make_work can still add (0 = 64) */
+    uint32_t max_launches; /* bound on the host launch loop (0 = none) */
+    uint32_t reserved;
+} hz_run_config;
+
+typedef struct hz_run_stats {
+    uint64_t promises;        /* promises evaluated */
+    uint64_t launches;        /* kernel launches of the exec loop */
+    uint64_t spills;          /* async: links moved LDS -> HBM */
+    uint64_t fills;           /* async: links moved HBM -> LDS */
+    uint64_t make_work_calls; /* wavefront calls of make_work */
+    uint32_t groups;          /* groups actually launched */
+    uint32_t reserved;
+    double elapsed_ms;        /* hipEvent time of the exec loop */
+} hz_run_stats;
+
+/* Runs the Collatz program over `count` starts (d_starts may be NULL: start
+ * = index) and writes d_steps[count]. kind = HZ_EVENT, HZ_ASYNC or HZ_PLAIN. */
+int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count, uint32_t* d_steps,
+                   const hz_run_config* config, hz_run_stats* stats);
+
+/* Fills d_out[count] with starts drawn uniformly from [1, 2^bits) by
+ * Philox4x32-10 keyed by seed: start i = (((x1 << 32) | x0) mod (2^bits - 1)) + 1
+ * where (x0, x1, x2, x3) = Philox4x32-10(counter = {i, 0, 0, 0},
+ * key = {seed & 0xFFFFFFFF, seed >> 32}). 1 <= bits <= 63. */
+int hz_collatz_make_starts(uint64_t seed, uint32_t bits, uint32_t count, uint64_t* d_out);
+
+/* Device facts used to size launches. */
+typedef struct hz_device_facts {
+    int32_t device;
+    int32_t compute_units;
+    int32_t max_groups_collatz_async; /* resident async groups per device */
+    int32_t max_groups_collatz_event;
+    int64_t total_memory;
+    char name[64];
+    char arch[64];
+} hz_device_facts;
+int hz_device_query(hz_device_facts* facts);
+
+int hz_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HARMONIZE_AMD_H */

@@ -1,0 +1,226 @@
+// ORACLE — test infrastructure only (see host_shim.h).
+//
+// HostProgram<Spec>: a sequential host executor of a program specification,
+// with the same device-side API as the GPU program types (prog.device,
+// prog.group, prog.thread, async<Op>, sync<Op>, load_fraction<Op>,
+// halt_early), so an unmodified specification runs on the CPU. It restates
+// the execution contract of the reference's program types:
+//  - initialize once, then evaluate queued promises; when none is queued,
+//    call make_work until it returns false (program/async.h:2201-2222,
+//    program/event.h:445-472); then finalize (async.h:2656, event.h:493);
+//  - an operation is dispatched by its discriminant, the position of its
+//    type in OpSet (async/promise.h:65-91, :647-658);
+//  - every promise is evaluated exactly once; sync<Op> evaluates in place
+//    (program/event.h:361-367).
+// Promises are evaluated first-in first-out, one at a time.
+//
+// OmpProgram<Spec>: the same contract on all host threads. Each thread has
+// a private queue and its own group/thread state; surplus promises move to
+// a shared pool that idle threads draw from; the run ends when every
+// thread is idle, the pool is empty and no thread can make more work.
+#pragma once
+
+#include <omp.h>
+
+#include <atomic>
+#include <cmath>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "host_shim.h"
+#include "harmonize/promise.h"
+#include "harmonize/iter.h"
+#include "harmonize/spec.h"
+
+namespace oracle {
+
+struct RunStats {
+    uint64_t promises = 0;
+    uint64_t make_work_calls = 0;
+    uint64_t peak_queue = 0;
+    uint64_t halts = 0;
+};
+
+template <typename SPEC>
+class HostProgram {
+   public:
+    typedef HostProgram<SPEC> ProgramType;
+    using OpSet = typename hz::spec::OpSetOr<SPEC, OpUnion<>>::type;
+    using DeviceState = typename hz::spec::DeviceStateOr<SPEC, VoidState>::type;
+    using GroupState = typename hz::spec::GroupStateOr<SPEC, VoidState>::type;
+    using ThreadState = typename hz::spec::ThreadStateOr<SPEC, VoidState>::type;
+    typedef PromiseUnion<OpSet> PromiseUnionType;
+    using Entry = std::pair<OpDisc, PromiseUnionType>;
+
+    template <typename TYPE>
+    struct Lookup {
+        typedef OpUnionLookup<TYPE, OpSet> type;
+    };
+
+    struct Runner {
+        std::deque<Entry> queue;
+        RunStats stats;
+    };
+
+    DeviceState& device;
+    GroupState& group;
+    ThreadState& thread;
+    Runner& _run;
+
+    HostProgram(DeviceState& d, GroupState& g, ThreadState& t, Runner& r)
+        : device(d), group(g), thread(t), _run(r) {}
+
+    template <typename TYPE, typename... ARGS>
+    void async(ARGS... args) {
+        async_call<TYPE>(Promise<TYPE>(args...));
+    }
+
+    template <typename TYPE>
+    void async_call(Promise<TYPE> promise) {
+        static_assert(Lookup<TYPE>::type::CONTAINED, "operation is not part of this program's OpSet");
+        PromiseUnionType pu;
+        pu.template cast<TYPE>() = promise;
+        _run.queue.emplace_back(Lookup<TYPE>::type::DISC, pu);
+        if (_run.queue.size() > _run.stats.peak_queue) _run.stats.peak_queue = _run.queue.size();
+    }
+
+    template <typename TYPE, typename... ARGS>
+    void sync(ARGS... args) {
+        Promise<TYPE>(args...)(*this);
+    }
+
+    template <typename TYPE>
+    void sync_call(Promise<TYPE> promise) {
+        promise(*this);
+    }
+
+    // A host executor has no bounded output queue; report it empty.
+    template <typename TYPE>
+    float load_fraction() {
+        return 0.0f;
+    }
+
+    // A GPU pass halted early is resumed by the host loop; the sequential
+    // executor therefore just records the request and carries on.
+    void halt_early() { _run.stats.halts++; }
+
+    // Evaluates one queued promise; false if the queue is empty.
+    bool step() {
+        if (_run.queue.empty()) return false;
+        Entry e = _run.queue.front();
+        _run.queue.pop_front();
+        e.second.template loose_eval<ProgramType>(*this, e.first);
+        _run.stats.promises++;
+        return true;
+    }
+
+    // Runs a specification to completion on one host thread.
+    static RunStats run(DeviceState state) {
+        Runner r;
+        GroupState g{};
+        ThreadState t{};
+        ProgramType prog(state, g, t, r);
+        SPEC::initialize(prog);
+        bool can_make = true;
+        while (true) {
+            if (prog.step()) continue;
+            if (can_make) {
+                r.stats.make_work_calls++;
+                can_make = SPEC::make_work(prog);
+                continue;
+            }
+            break;
+        }
+        SPEC::finalize(prog);
+        return r.stats;
+    }
+};
+
+template <typename SPEC>
+struct OmpProgram {
+    using Host = HostProgram<SPEC>;
+    using DeviceState = typename Host::DeviceState;
+    using GroupState = typename Host::GroupState;
+    using ThreadState = typename Host::ThreadState;
+    using Entry = typename Host::Entry;
+
+    static constexpr size_t SHARE_ABOVE = 4096;  // local queue length that triggers sharing
+    static constexpr size_t TAKE = 1024;         // promises taken from the pool at once
+
+    static RunStats run(DeviceState state, int threads) {
+        if (threads <= 0) threads = omp_get_max_threads();
+        std::mutex lock;
+        std::vector<Entry> pool;
+        int busy = threads;
+        RunStats total;
+
+#pragma omp parallel num_threads(threads)
+        {
+            typename Host::Runner r;
+            GroupState g{};
+            ThreadState t{};
+            DeviceState local_state = state;
+            Host prog(local_state, g, t, r);
+            SPEC::initialize(prog);
+            bool can_make = true;
+            while (true) {
+                if (prog.step()) {
+                    if (r.queue.size() > SHARE_ABOVE) {
+                        std::lock_guard<std::mutex> guard(lock);
+                        size_t n = r.queue.size() / 2;
+                        for (size_t i = 0; i < n; ++i) {
+                            pool.push_back(r.queue.back());
+                            r.queue.pop_back();
+                        }
+                    }
+                    continue;
+                }
+                if (can_make) {
+                    r.stats.make_work_calls++;
+                    can_make = SPEC::make_work(prog);
+                    continue;
+                }
+                // Idle: draw from the pool, or finish once nobody can produce work.
+                bool done = false;
+                bool idle_marked = false;
+                while (true) {
+                    {
+                        std::lock_guard<std::mutex> guard(lock);
+                        if (!pool.empty()) {
+                            size_t n = pool.size() < TAKE ? pool.size() : TAKE;
+                            for (size_t i = 0; i < n; ++i) {
+                                r.queue.push_back(pool.back());
+                                pool.pop_back();
+                            }
+                            if (idle_marked) busy++;
+                            break;
+                        }
+                        if (!idle_marked) {
+                            busy--;
+                            idle_marked = true;
+                        }
+                        if (busy == 0) {
+                            done = true;
+                            break;
+                        }
+                    }
+                    std::this_thread::yield();
+                }
+                if (done) break;
+            }
+            SPEC::finalize(prog);
+            std::lock_guard<std::mutex> guard(lock);
+            total.promises += r.stats.promises;
+            total.make_work_calls += r.stats.make_work_calls;
+            total.halts += r.stats.halts;
+            if (r.stats.peak_queue > total.peak_queue) total.peak_queue = r.stats.peak_queue;
+        }
+        return total;
+    }
+
+};
+
+}  // namespace oracle

@@ -1,0 +1,122 @@
+// ORACLE — test infrastructure only (see host_shim.h). Built into
+// oracle/liboracle.so and called from tests, the smoke check and the
+// bench's CPU-baseline leg through ctypes.
+#include <omp.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+
+#include "host_program.h"
+#include "collatz.h"
+#include "philox.h"
+
+extern "C" {
+
+// Collatz step count of v with the runtime's overflow policy: the closed
+// form of the reference's host checker (examples/cpp/collaz/collaz_hrm.cpp:
+// 157-169; collaz_std.cpp:28-40), evaluated in 128-bit arithmetic so that a
+// trajectory leaving 64 bits is detected exactly rather than wrapping.
+// `evaluations` receives the number of promises the Collatz program
+// evaluates for this start: one per step plus the final one, where a
+// trajectory that overflows ends at the step whose 3v+1 would overflow.
+static uint32_t collatz_closed_form(uint64_t start, uint64_t* evaluations) {
+    unsigned __int128 v = start;
+    const unsigned __int128 top = ((unsigned __int128)1 << 64);
+    uint32_t steps = 0;
+    while (v > 1) {
+        if ((v & 1) == 0) {
+            v >>= 1;
+        } else {
+            v = 3 * v + 1;
+            if (v >= top) {
+                *evaluations = (uint64_t)steps + 1u;
+                return COLLATZ_OVERFLOW;
+            }
+        }
+        steps++;
+    }
+    *evaluations = (uint64_t)steps + 1u;
+    return steps;
+}
+
+uint32_t oracle_collatz_steps_one(uint64_t start) {
+    uint64_t evaluations = 0;
+    return collatz_closed_form(start, &evaluations);
+}
+
+// Closed form over many starts (starts may be NULL: start = index), on
+// `threads` OpenMP threads (0 = all). Returns the number of promises the
+// Collatz program evaluates over all starts.
+uint64_t oracle_collatz_steps(const uint64_t* starts, uint32_t count, uint32_t* steps, int threads) {
+    if (threads <= 0) threads = omp_get_max_threads();
+    uint64_t promises = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : promises)
+    for (int64_t i = 0; i < (int64_t)count; ++i) {
+        uint64_t s = starts ? starts[i] : (uint64_t)i;
+        uint64_t e = 0;
+        steps[i] = collatz_closed_form(s, &e);
+        promises += e;
+    }
+    return promises;
+}
+
+struct oracle_run_stats {
+    uint64_t promises;
+    uint64_t make_work_calls;
+    uint64_t peak_queue;
+    double elapsed_ms;
+    int32_t threads;
+    int32_t reserved;
+};
+
+static void fill_stats(oracle_run_stats* out, const oracle::RunStats& s, double ms, int threads) {
+    if (!out) return;
+    out->promises = s.promises;
+    out->make_work_calls = s.make_work_calls;
+    out->peak_queue = s.peak_queue;
+    out->elapsed_ms = ms;
+    out->threads = threads;
+    out->reserved = 0;
+}
+
+// Runs the Collatz specification itself on the sequential host executor
+// (threads == 1) or the OpenMP executor (threads != 1; 0 = all cores).
+int oracle_collatz_program(const uint64_t* starts, uint32_t count, uint32_t* steps, int threads,
+                           oracle_run_stats* stats) {
+    util::iter::AtomicIter<unsigned int> iter(0u, count);
+    CollatzState ds{(const unsigned long long*)starts, steps, &iter};
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s;
+    int used = 1;
+    if (threads == 1) {
+        s = oracle::HostProgram<CollatzSpec>::run(ds);
+    } else {
+        used = threads <= 0 ? omp_get_max_threads() : threads;
+        s = oracle::OmpProgram<CollatzSpec>::run(ds, used);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Philox4x32-10 of one counter under one key.
+void oracle_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    hz::philox::Ctr r = hz::philox::philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    std::memcpy(out, r.v, sizeof(r.v));
+}
+
+// Same start sequence as hz_collatz_make_starts (declared in harmonize_amd.h).
+void oracle_collatz_make_starts(uint64_t seed, uint32_t bits, uint32_t count, uint64_t* out) {
+    const uint64_t modulus = (1ull << bits) - 1ull;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)count; ++i) {
+        hz::philox::Ctr r = hz::philox::philox4x32_10((uint32_t)i, 0u, 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        uint64_t x = ((uint64_t)r.v[1] << 32) | r.v[0];
+        out[i] = x % modulus + 1ull;
+    }
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
