@@ -1,9 +1,9 @@
 // EventProgram<Spec>: the level-synchronous program type.
 //
-// Semantics follow the reference's program/event.h:383-518: one kernel launch
+// Semantics follow the reference's program/event.h:383-518: one launch
 // evaluates one generation. Each group drains the per-operation input queues
 // in chunks; a group that finds no input calls the spec's make_work until
-// make_work returns false or some output queue is half full; then every queue
+// make_work returns false or an output queue is half full; then every queue
 // is flipped (outputs become the next generation's inputs). The host loops
 // `exec` until a generation produces no promises.
 //
@@ -12,13 +12,20 @@
 //  - queues hold promises as 32-bit words in structure-of-arrays form
 //    ([op][word][capacity]), so a wavefront reads or writes 64 consecutive
 //    words per row: fully coalesced 256-byte accesses;
-//  - an async call costs one global atomic per wavefront per call site
-//    (ballot + mbcnt), not one atomic and two system fences per lane as in
-//    the reference (util/iter.h:197-214);
-//  - inputs are claimed in chunks of chunk*64 promises with one atomic;
-//  - the flip runs in a one-group kernel launched after each generation
-//    (the reference's last group checks out through a counter that every
-//    group increments, which serialises thousands of atomics per launch).
+//  - each operation's queue is split into SEGMENTS equal segments, each with
+//    its own counters on its own 128-byte line. A group pushes to, and first
+//    drains, its home segment (blockIdx % SEGMENTS) and moves on to other
+//    segments only when its own is full or empty. Device-scope atomics on one
+//    address serialise across the XCDs at a few tens of ns each, so a single
+//    counter per queue would cap a generation at tens of millions of pushes
+//    per second;
+//  - an async call costs one atomic per wavefront per call site (ballot +
+//    mbcnt), not one atomic and two system fences per lane as in the
+//    reference (util/iter.h:197-214); inputs are claimed chunk*64 at a time;
+//  - the flip runs in a one-group kernel after each generation (the
+//    reference's last group checks out through a counter that every group
+//    increments). It also publishes a bitmask of the non-empty input
+//    segments, so sparse generations do not scan every counter.
 #pragma once
 
 #include "host.h"
@@ -57,7 +64,9 @@ class EventProgram {
 
     static const unsigned char FN_ID_COUNT = PromiseUnionType::Info::COUNT;
     static constexpr uint32_t OPS = PromiseUnionType::Info::COUNT;
+    static constexpr uint32_t OPS1 = OPS > 0 ? OPS : 1;
     static constexpr uint32_t WORDS = hz::PromiseWords<OpSet>::value;
+    static constexpr uint32_t SEGMENTS = 64;
 
     static_assert(std::is_trivially_copyable<PromiseUnionType>::value,
                   "promise arguments must be trivially copyable");
@@ -69,17 +78,25 @@ class EventProgram {
 
     struct GroupContext {
         unsigned int toggle;
+        unsigned long long int nonempty[OPS1];
+    };
+
+    // Counters of one segment of one operation's queue (one 128-byte line).
+    struct Segment {
+        unsigned int in_count;   // promises in this segment's input
+        unsigned int in_claim;   // input claimed so far
+        unsigned int out_count;  // output slots claimed (may pass the capacity)
+        unsigned int pad[29];
     };
 
     // Queue counters and run status, kept in HBM.
     struct Status {
-        unsigned int in_count[OPS > 0 ? OPS : 1];
-        unsigned int in_claim[OPS > 0 ? OPS : 1];
-        unsigned int out_count[OPS > 0 ? OPS : 1];
+        Segment seg[OPS1][SEGMENTS];
+        unsigned long long int nonempty[OPS1];  // bit s: input segment s is non-empty
         unsigned int toggle;
-        unsigned int checkout;
         unsigned int flags;
         unsigned int flip_count;
+        unsigned int pad;
         unsigned long long int promises;
         unsigned long long int make_work_calls;
     };
@@ -88,31 +105,36 @@ class EventProgram {
         typedef ProgramType ParentProgramType;
         Status* status;
         unsigned int* queues[2];  // each [OPS][WORDS][capacity]
-        unsigned int capacity;
-        unsigned int group_count;
+        unsigned int capacity;    // per operation, SEGMENTS * seg_capacity
+        unsigned int seg_capacity;
     };
 
     // Owns the queues and status of one program instance (reference
     // program/event.h:139-230). `io_size` is the per-operation capacity of
-    // each queue, in promises.
+    // each queue, in promises (rounded up to a multiple of SEGMENTS).
     struct Instance {
+        unsigned int seg_capacity;
+        unsigned int capacity;
         util::host::DevBuf<Status> status;
         util::host::DevBuf<unsigned int> queue_a;
         util::host::DevBuf<unsigned int> queue_b;
         DeviceState device_state;
-        unsigned int capacity;
         Status host_status{};
-        unsigned int last_groups = 0;
 
-        Instance(size_t io_size, DeviceState gs)
-            : status((size_t)1),
-              queue_a((size_t)OPS * WORDS * (size_t)io_size),
-              queue_b((size_t)OPS * WORDS * (size_t)io_size),
-              device_state(gs),
-              capacity((unsigned int)io_size) {
+        static unsigned int seg_cap_for(size_t io_size) {
             if (io_size == 0 || io_size > 0x7FFFFFFFu) {
                 throw std::runtime_error("EventProgram::Instance: io_size must be in [1, 2^31)");
             }
+            return (unsigned int)((io_size + SEGMENTS - 1) / SEGMENTS);
+        }
+
+        Instance(size_t io_size, DeviceState gs)
+            : seg_capacity(seg_cap_for(io_size)),
+              capacity(seg_capacity * SEGMENTS),
+              status((size_t)1),
+              queue_a((size_t)OPS1 * WORDS * (size_t)seg_capacity * SEGMENTS),
+              queue_b((size_t)OPS1 * WORDS * (size_t)seg_capacity * SEGMENTS),
+              device_state(gs) {
             util::host::auto_throw(hipMemset(status.data(), 0, sizeof(Status)));
         }
 
@@ -122,7 +144,7 @@ class EventProgram {
             result.queues[0] = queue_a.data();
             result.queues[1] = queue_b.data();
             result.capacity = capacity;
-            result.group_count = last_groups;
+            result.seg_capacity = seg_capacity;
             return result;
         }
 
@@ -131,22 +153,30 @@ class EventProgram {
             return host_status;
         }
 
-        // True once a generation has produced no promises. Raises if any
+        // Reads only the flags word (the per-generation completion check).
+        unsigned int read_flags() {
+            unsigned int flags = 0;
+            util::host::auto_throw(
+                hipMemcpy(&flags, &status.data()->flags, sizeof(flags), hipMemcpyDeviceToHost));
+            host_status.flags = flags;
+            return flags;
+        }
+
+        // True once a generation has produced no promises. Raises if an
         // output queue overflowed (promises would have been lost).
         bool complete() {
-            read_status();
-            if (host_status.flags & OVERFLOW_FLAG) {
+            unsigned int flags = read_flags();
+            if (flags & OVERFLOW_FLAG) {
                 throw std::runtime_error(
                     "EventProgram: an output queue overflowed; construct the Instance with a larger io_size");
             }
-            return (host_status.flags & COMPLETION_FLAG) != 0;
+            return (flags & COMPLETION_FLAG) != 0;
         }
 
         void clear_flags() {
-            read_status();
-            host_status.flags = 0;
-            util::host::auto_throw(hipMemcpy(&status.data()->flags, &host_status.flags, sizeof(unsigned int),
-                                             hipMemcpyHostToDevice));
+            unsigned int zero = 0;
+            util::host::auto_throw(
+                hipMemcpy(&status.data()->flags, &zero, sizeof(unsigned int), hipMemcpyHostToDevice));
         }
 
         unsigned long long int promises() { return read_status().promises; }
@@ -169,10 +199,15 @@ class EventProgram {
         return _dev_ctx.queues[which] + ((size_t)op * WORDS + word) * _dev_ctx.capacity;
     }
 
+    __device__ Segment& segment(OpDisc op, uint32_t s) { return _dev_ctx.status->seg[op][s]; }
+
+    __device__ uint32_t home_segment() const { return blockIdx.x % SEGMENTS; }
+
     __device__ void set_flags(unsigned int bits) { hz::wave::atomic_or(&_dev_ctx.status->flags, bits); }
 
-    // Appends one promise per active lane to the output queue of TYPE with a
-    // single atomic for the wavefront.
+    // Appends one promise per active lane to the output queue of TYPE: one
+    // atomic per wavefront on the home segment, spilling to the following
+    // segments only when it is full.
     template <typename TYPE>
     __device__ void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
         constexpr OpDisc D = Lookup<TYPE>::type::DISC;
@@ -180,21 +215,30 @@ class EventProgram {
         PromiseUnionType pu;
         pu.template cast<TYPE>() = param_value;
         uint32_t words[WORDS];
-        __builtin_memcpy(words, &pu, sizeof(PromiseUnionType) < sizeof(words) ? sizeof(PromiseUnionType)
-                                                                                : sizeof(words));
-        uint64_t act = hz::wave::active();
-        uint32_t r = hz::wave::rank(act);
-        uint32_t base = 0;
-        if (r == 0) base = hz::wave::atomic_add(&_dev_ctx.status->out_count[D], hz::wave::popc(act));
-        base = hz::wave::first(base);
-        uint32_t idx = base + r;
-        if (idx < _dev_ctx.capacity) {
-            unsigned int out = _grp_ctx.toggle ^ 1u;
+        __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
+        const unsigned int out = _grp_ctx.toggle ^ 1u;
+        const uint32_t cap = _dev_ctx.seg_capacity;
+        bool pending = true;
+        uint32_t s = home_segment();
+        for (uint32_t attempt = 0; attempt < SEGMENTS; ++attempt) {
+            uint64_t want = hz::wave::ballot(pending);
+            if (want == 0) break;
+            uint32_t r = hz::wave::rank(want);
+            uint32_t base = 0;
+            if (pending && r == 0) base = hz::wave::atomic_add(&segment(D, s).out_count, hz::wave::popc(want));
+            base = hz::wave::first(hz::wave::read_lane(base, (uint32_t)__builtin_ctzll(want)));
+            if (pending) {
+                uint32_t idx = base + r;
+                if (idx < cap) {
+                    size_t at = (size_t)s * cap + idx;
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) queue_row(out, D, w)[idx] = words[w];
-        } else {
-            set_flags(OVERFLOW_FLAG);
+                    for (uint32_t w = 0; w < WORDS; ++w) queue_row(out, D, w)[at] = words[w];
+                    pending = false;
+                }
+            }
+            s = (s + 1 == SEGMENTS) ? 0 : s + 1;
         }
+        if (hz::wave::ballot(pending) != 0 && hz::wave::leader()) set_flags(OVERFLOW_FLAG);
     }
 
     template <typename TYPE>
@@ -208,6 +252,20 @@ class EventProgram {
         pu.template loose_eval<ProgramType>(*this, op);
     }
 
+    // Claims up to `span` input promises of operation `op` in segment `s`.
+    __device__ bool claim(uint32_t op, uint32_t s, uint32_t span, uint32_t& base, uint32_t& end) {
+        Segment& g = segment(op, s);
+        uint32_t limit = hz::wave::first(hz::wave::atomic_load(&g.in_count));
+        if (hz::wave::first(hz::wave::atomic_load(&g.in_claim)) >= limit) return false;
+        uint32_t b = 0;
+        if (hz::wave::lane() == 0) b = hz::wave::atomic_add(&g.in_claim, span);
+        b = hz::wave::first(b);
+        if (b >= limit) return false;
+        base = s * _dev_ctx.seg_capacity + b;
+        end = s * _dev_ctx.seg_capacity + ((limit - b > span) ? b + span : limit);
+        return true;
+    }
+
    public:
     __device__ void init_program() {}
 
@@ -216,38 +274,51 @@ class EventProgram {
         const uint32_t lane = hz::wave::lane();
         _thd_ctx.thread_id = blockIdx.x * 64u + lane;
         _thd_ctx.rand_state = _thd_ctx.thread_id;
-        if (lane == 0) _grp_ctx.toggle = hz::wave::atomic_load(&_dev_ctx.status->toggle);
+        Status* st = _dev_ctx.status;
+        if (lane == 0) {
+            _grp_ctx.toggle = hz::wave::atomic_load(&st->toggle);
+            for (uint32_t i = 0; i < OPS; ++i) _grp_ctx.nonempty[i] = hz::wave::atomic_load(&st->nonempty[i]);
+        }
         hz::wave::sync();
         const unsigned int in = hz::wave::first(_grp_ctx.toggle);
         const unsigned int span = (chunk_size == 0 ? 1u : chunk_size) * 64u;
-        Status* st = _dev_ctx.status;
+        const uint32_t home = home_segment();
         unsigned long long made = 0;
 
         PROGRAM_SPEC::initialize(*this);
 
         while (true) {
-            // Claim a chunk of input from the first operation that has any.
+            // Claim a chunk: home segment first, then the other non-empty ones.
             uint32_t op = OPS, base = 0, end = 0;
-            for (uint32_t i = 0; i < OPS; ++i) {
-                uint32_t limit = hz::wave::first(hz::wave::atomic_load(&st->in_count[i]));
-                if (hz::wave::first(hz::wave::atomic_load(&st->in_claim[i])) >= limit) continue;
-                uint32_t b = 0;
-                if (lane == 0) b = hz::wave::atomic_add(&st->in_claim[i], span);
-                b = hz::wave::first(b);
-                if (b < limit) {
-                    op = i;
-                    base = b;
-                    end = (limit - b > span) ? b + span : limit;
-                    break;
+            for (uint32_t i = 0; i < OPS && op == OPS; ++i) {
+                uint64_t live = hz::wave::first((uint64_t)_grp_ctx.nonempty[i]);
+                if (live == 0) continue;
+                if ((live >> home) & 1ull) {
+                    if (claim(i, home, span, base, end)) {
+                        op = i;
+                        break;
+                    }
+                    live &= ~(1ull << home);
                 }
+                while (live != 0) {
+                    uint32_t s = (uint32_t)__builtin_ctzll(live);
+                    if (claim(i, s, span, base, end)) {
+                        op = i;
+                        break;
+                    }
+                    live &= live - 1;
+                }
+                if (op == OPS && hz::wave::lane() == 0) _grp_ctx.nonempty[i] = 0;
+                hz::wave::sync();
             }
             if (op == OPS) {
                 // No input left for this group: make work until the spec says
-                // it has none or an output queue is half full.
-                bool more = (hz::wave::atomic_load(&st->flags) & EARLY_HALT_FLAG) == 0;
+                // it has none or this group's output segment is half full.
+                bool more = (hz::wave::first(hz::wave::atomic_load(&st->flags)) & EARLY_HALT_FLAG) == 0;
                 while (more) {
                     for (uint32_t i = 0; i < OPS; ++i) {
-                        if (hz::wave::atomic_load(&st->out_count[i]) >= _dev_ctx.capacity / 2) more = false;
+                        uint32_t n = hz::wave::first(hz::wave::atomic_load(&segment(i, home).out_count));
+                        if (n >= _dev_ctx.seg_capacity / 2) more = false;
                     }
                     if (!more) break;
                     bool mine = PROGRAM_SPEC::make_work(*this);
@@ -265,35 +336,40 @@ class EventProgram {
         }
 
         PROGRAM_SPEC::finalize(*this);
-
-        // The queues are flipped by flip_kernel after this launch: the kernel
-        // boundary orders every group's pushes before the flip, so no group
-        // has to check out through a contended counter.
         if (lane == 0 && made) hz::wave::atomic_add(&st->make_work_calls, made);
     }
 
     // Flips every queue for the next generation (reference event.h:500-515,
     // util/iter.h:437-445) and records completion when nothing was produced.
-    // Launched as one 64-lane group after each generation.
+    // Launched as one 64-lane group after each generation; lane s handles
+    // segment s of every operation.
+    static_assert(SEGMENTS == 64, "flip assigns one segment per lane");
     __device__ static void flip(DeviceContext ctx) {
-        if (hz::wave::lane() != 0) return;
         Status* st = ctx.status;
-        unsigned long long executed = 0, produced_total = 0;
+        const uint32_t lane = hz::wave::lane();
+        unsigned long long executed = 0, produced = 0;
         for (uint32_t i = 0; i < OPS; ++i) {
-            executed += st->in_count[i];
-            unsigned int produced = st->out_count[i];
-            unsigned int n = produced < ctx.capacity ? produced : ctx.capacity;
-            st->in_count[i] = n;
-            st->in_claim[i] = 0;
-            st->out_count[i] = 0;
-            produced_total += n;
+            Segment& g = st->seg[i][lane];
+            uint32_t done_here = g.in_count;
+            uint32_t n = g.out_count < ctx.seg_capacity ? g.out_count : ctx.seg_capacity;
+            g.in_count = n;
+            g.in_claim = 0;
+            g.out_count = 0;
+            uint64_t live = hz::wave::ballot(n != 0);
+            for (uint32_t l = 0; l < 64; ++l) {
+                executed += hz::wave::read_lane(done_here, l);
+                produced += hz::wave::read_lane(n, l);
+            }
+            if (lane == 0) st->nonempty[i] = live;
         }
-        st->promises += executed;
-        st->toggle ^= 1u;
-        st->flip_count += 1;
-        unsigned int flags = st->flags;
-        if (produced_total == 0 && (flags & EARLY_HALT_FLAG) == 0) flags |= COMPLETION_FLAG;
-        st->flags = flags & ~EARLY_HALT_FLAG;
+        if (lane == 0) {
+            st->promises += executed;
+            st->toggle ^= 1u;
+            st->flip_count += 1;
+            unsigned int flags = st->flags;
+            if (produced == 0 && (flags & EARLY_HALT_FLAG) == 0) flags |= COMPLETION_FLAG;
+            st->flags = flags & ~EARLY_HALT_FLAG;
+        }
     }
 
     template <typename TYPE, typename... ARGS>
@@ -316,16 +392,17 @@ class EventProgram {
         immediate_call_cast<TYPE>(promise);
     }
 
-    // Fill fraction of TYPE's output queue (reference event.h:540-544).
+    // Fill fraction of TYPE's output in this group's home segment (reference
+    // event.h:540-544 reports the whole queue's).
     template <typename TYPE>
     __device__ float load_fraction() {
-        unsigned int n = hz::wave::atomic_load(&_dev_ctx.status->out_count[Lookup<TYPE>::type::DISC]);
-        return (float)n / (float)_dev_ctx.capacity;
+        unsigned int n =
+            hz::wave::atomic_load(&segment(Lookup<TYPE>::type::DISC, home_segment()).out_count);
+        return (float)n / (float)_dev_ctx.seg_capacity;
     }
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
 
-    // Kernel bodies used by the launch helpers.
     __device__ static void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int chunk) {
         __shared__ GroupContext gctx;
         __shared__ GroupState group_state;

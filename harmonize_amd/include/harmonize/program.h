@@ -58,9 +58,6 @@ template <typename ProgType>
 __host__ void exec(typename ProgType::Instance& instance, size_t group_count, size_t n, hipStream_t stream = nullptr) {
     if (group_count == 0) group_count = 1;
     auto ctx = instance.to_context();
-    if constexpr (!hz::IsAsync<ProgType>::value) {
-        ctx.group_count = (unsigned int)group_count;
-    }
     hipLaunchKernelGGL(_hz_exec_kernel<ProgType>, dim3((unsigned)group_count), dim3(64), 0, stream, ctx,
                        instance.device_state, (unsigned int)(n > 0xFFFFFFFFull ? 0xFFFFFFFFull : n));
     if constexpr (!hz::IsAsync<ProgType>::value) {

@@ -1,0 +1,87 @@
+"""ctypes handle on oracle/liboracle.so, the CPU oracle (test infrastructure only)."""
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_PATH = ROOT / "oracle" / "liboracle.so"
+
+_VP = ctypes.c_void_p
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [
+        ("promises", ctypes.c_uint64),
+        ("make_work_calls", ctypes.c_uint64),
+        ("peak_queue", ctypes.c_uint64),
+        ("elapsed_ms", ctypes.c_double),
+        ("threads", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_SIGS = {
+    "oracle_collatz_steps_one": (ctypes.c_uint32, [ctypes.c_uint64]),
+    "oracle_collatz_steps": (ctypes.c_uint64, [_VP, ctypes.c_uint32, _VP, ctypes.c_int]),
+    "oracle_collatz_program": (ctypes.c_int, [_VP, ctypes.c_uint32, _VP, ctypes.c_int, ctypes.POINTER(OracleStats)]),
+    "oracle_philox4x32_10": (None, [_VP, _VP, _VP]),
+    "oracle_collatz_make_starts": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+    "oracle_max_threads": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_PATH.exists():
+            raise RuntimeError(f"{ORACLE_PATH} is not built; run `python -m harmonize_amd.build --oracle`")
+        h = ctypes.CDLL(str(ORACLE_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def collatz_steps(starts, count=None, threads=0):
+    """Closed-form step counts; returns (steps, promises)."""
+    n = len(starts) if starts is not None else int(count)
+    if starts is not None:
+        starts = np.ascontiguousarray(starts, dtype=np.uint64)
+    out = np.zeros(n, dtype=np.uint32)
+    promises = lib().oracle_collatz_steps(_ptr(starts), n, out.ctypes.data, threads)
+    return out, int(promises)
+
+
+def collatz_program(starts, count=None, threads=1):
+    """Runs the Collatz specification on the host executor; returns (steps, stats)."""
+    n = len(starts) if starts is not None else int(count)
+    if starts is not None:
+        starts = np.ascontiguousarray(starts, dtype=np.uint64)
+    out = np.zeros(n, dtype=np.uint32)
+    st = OracleStats()
+    lib().oracle_collatz_program(_ptr(starts), n, out.ctypes.data, threads, ctypes.byref(st))
+    return out, st
+
+
+def collatz_make_starts(seed, bits, count):
+    out = np.zeros(count, dtype=np.uint64)
+    lib().oracle_collatz_make_starts(seed, bits, count, out.ctypes.data)
+    return out
+
+
+def philox(counter, key):
+    c = np.asarray(counter, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().oracle_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return [int(x) for x in o]
