@@ -51,10 +51,16 @@ class RunStats(ctypes.Structure):
         ("groups", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
         ("elapsed_ms", ctypes.c_double),
+        ("phase_cycles", ctypes.c_uint64 * 8),
     ]
 
+    PHASES = ("cycle", "spill", "pull", "make_work", "pick", "read", "eval", "idle")
+
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        out = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("reserved", "phase_cycles")}
+        if any(self.phase_cycles):
+            out["phase_cycles"] = dict(zip(self.PHASES, [int(x) for x in self.phase_cycles]))
+        return out
 
 
 class DeviceFacts(ctypes.Structure):
@@ -227,3 +233,81 @@ def collatz_make_starts(seed: int, bits: int, count: int) -> DeviceArray:
     out = DeviceArray(count, np.uint64)
     check(lib().hz_collatz_make_starts(seed, bits, count, out.ptr), "hz_collatz_make_starts")
     return out
+
+
+# ------------------------------------------------------------------ photon walk
+
+PHOTON_EXTRA_SLOTS = 5
+PHOTON_SLOTS = ("reflected", "transmitted", "roulette_lost", "roulette_gained", "capped")
+
+
+class PhotonParams(ctypes.Structure):
+    _fields_ = [
+        ("photons", ctypes.c_uint32),
+        ("bins", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("thickness", ctypes.c_float),
+        ("sigma_t", ctypes.c_float),
+        ("albedo", ctypes.c_float),
+        ("split_above", ctypes.c_float),
+        ("roulette_below", ctypes.c_float),
+        ("region_width", ctypes.c_float),
+        ("regions", ctypes.c_uint32),
+        ("max_split", ctypes.c_uint32),
+        ("max_events", ctypes.c_uint32),
+        ("replicas", ctypes.c_uint32),
+    ]
+
+
+_SIGNATURES.update(
+    {
+        "hz_photon_run": (
+            ctypes.c_int,
+            [ctypes.c_int, ctypes.POINTER(PhotonParams), _VP, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
+        ),
+        "hz_photon_defaults": (None, [ctypes.POINTER(PhotonParams)]),
+        "hz_detmath_eval": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_uint32]),
+    }
+)
+for _s in ("photon_event", "photon_async"):
+    _SIGNATURES.update(
+        {
+            f"alloc_state_{_s}": (_VP, []),
+            f"free_state_{_s}": (None, [_VP]),
+            f"load_state_device_{_s}": (None, [_VP, _VP]),
+            f"store_state_device_{_s}": (None, [_VP, _VP]),
+            f"alloc_program_{_s}": (_VP, [_VP, ctypes.c_size_t]),
+            f"free_program_{_s}": (None, [_VP]),
+            f"init_program_{_s}": (None, [_VP, ctypes.c_size_t]),
+            f"exec_program_{_s}": (None, [_VP, ctypes.c_size_t, ctypes.c_size_t]),
+            f"complete_{_s}": (ctypes.c_int, [_VP]),
+            f"clear_flags_{_s}": (None, [_VP]),
+            f"set_device_{_s}": (None, [ctypes.c_int]),
+        }
+    )
+
+
+def photon_params(**overrides) -> PhotonParams:
+    """The photon-walk defaults (1e7 photons, slab 20, albedo 0.9, ...) with overrides."""
+    p = PhotonParams()
+    lib().hz_photon_defaults(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def photon_run(kind, params: PhotonParams, tally: DeviceArray | None = None, events: DeviceArray | None = None,
+               **cfg):
+    """Runs the photon walk; returns (summed tally, events per photon, stats)."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    stride = params.bins + PHOTON_EXTRA_SLOTS
+    if tally is None:
+        tally = DeviceArray(params.replicas * stride, np.float64)
+    if events is None:
+        events = DeviceArray(params.photons, np.uint32)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(lib().hz_photon_run(k, ctypes.byref(params), tally.ptr, events.ptr, ctypes.byref(config), ctypes.byref(st)),
+          "hz_photon_run")
+    reps = tally.numpy().reshape(params.replicas, stride)
+    return reps.sum(axis=0), events.numpy(), st.as_dict()

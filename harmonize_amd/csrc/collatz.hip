@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256)
 namespace {
 
 uint32_t default_groups_async() {
-    return (uint32_t)hz::capi::resident_groups(_hz_exec_kernel<CollatzAsync>);
+    return (uint32_t)hz::capi::resident_groups_of<CollatzAsync>();
 }
 
 uint32_t default_groups_event() {
@@ -145,6 +145,6 @@ extern "C" int hz_device_query(hz_device_facts* facts) {
         std::strncpy(facts->name, prop.name, sizeof(facts->name) - 1);
         std::strncpy(facts->arch, prop.gcnArchName, sizeof(facts->arch) - 1);
         facts->max_groups_collatz_async = (int32_t)default_groups_async();
-        facts->max_groups_collatz_event = (int32_t)hz::capi::resident_groups(_hz_exec_kernel<CollatzEvent>);
+        facts->max_groups_collatz_event = (int32_t)hz::capi::resident_groups_of<CollatzEvent>();
     });
 }

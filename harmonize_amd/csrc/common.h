@@ -40,13 +40,20 @@ inline void check(hipError_t e, const char* what) {
 
 int compute_units();
 
-// Groups (64-lane workgroups) that fit on the device at once for `kernel`.
+// Groups (64-lane wavefronts) that fit on the device at once for `kernel`,
+// launched in workgroups of `block_threads` threads.
 template <typename K>
-int resident_groups(K kernel) {
+int resident_groups(K kernel, int block_threads = 64) {
     int per_cu = 0;
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0), "occupancy query");
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0), "occupancy query");
     if (per_cu < 1) per_cu = 1;
-    return per_cu * compute_units();
+    return per_cu * compute_units() * (block_threads / 64);
+}
+
+// The same for a program type's exec kernel.
+template <typename P>
+int resident_groups_of() {
+    return resident_groups(_hz_exec_kernel<P>, (int)P::WORK_GROUP_SIZE);
 }
 
 // Owns a device AtomicIter<unsigned int> set to [0, limit).
@@ -119,8 +126,9 @@ void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uin
         stats->spills = s.spills;
         stats->fills = s.fills;
         stats->make_work_calls = s.make_work_calls;
-        stats->groups = groups;
+        stats->groups = (uint32_t)(blocks_for<P>(groups) * (P::WORK_GROUP_SIZE / 64));
         stats->elapsed_ms = ms;
+        for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
     }
 }
 

@@ -21,7 +21,8 @@
 //    a time, async.h:1207, :1767-1771);
 //  - the HBM arena holds links in the same [link][word][lane] layout;
 //    full links travel through bounded lock-free ring queues of link ids
-//    (one queue per XCD by default, selected from blockIdx % 8), with
+//    (one queue per XCD by default, selected from blockIdx % 8, since
+//    workgroups are dealt to the 8 XCDs round-robin), with
 //    stealing from other queues when a group's own queue is empty;
 //  - termination is counter-based: `units` counts groups that hold promises
 //    plus links parked in queues, `makers` counts groups whose make_work may
@@ -62,6 +63,22 @@ struct AsyncStatus {
     unsigned long long int make_work_calls;
     unsigned long long int launches;
     unsigned long long int idle_polls;
+    // Shader-clock cycles per scheduler phase, summed over groups; filled only
+    // when the runtime is built with HZ_PHASE_TIMING (the counterpart of the
+    // reference's HRM_TIME phase timers, preamble/mod.h:59-67).
+    unsigned long long int phase_cycles[8];
+};
+
+// Scheduler phases timed under HZ_PHASE_TIMING.
+enum AsyncPhase : unsigned int {
+    PHASE_CYCLE = 0,   // a whole scheduling cycle
+    PHASE_SPILL = 1,   // keeping free stash links
+    PHASE_PULL = 2,    // pulling queued links
+    PHASE_MAKE = 3,    // make_work
+    PHASE_PICK = 4,    // choosing the next link
+    PHASE_READ = 5,    // reading a link into registers
+    PHASE_EVAL = 6,    // evaluating promises (spec code + async calls)
+    PHASE_IDLE = 7,    // idle polling
 };
 
 constexpr unsigned int LINK_NONE = 0xFFFFFFFFu;
@@ -98,7 +115,16 @@ class AsyncProgram {
     static const size_t POOL_SIZE = hz::spec::POOL_SIZEOr<PROGRAM_SPEC, 32>::value;
     static const size_t STACK_SIZE = hz::spec::STACK_SIZEOr<PROGRAM_SPEC, 0>::value;
     static const size_t GROUP_SIZE = 64;
-    static const size_t WORK_GROUP_SIZE = 64;
+    // Groups per workgroup: each group keeps its own stash and scheduler
+    // state; the groups of a workgroup share its __shared__ memory, so a
+    // specification's block-level LDS data is paid for once per workgroup.
+    static constexpr uint32_t GROUPS_PER_BLOCK =
+        (uint32_t)hz::spec::GROUPS_PER_BLOCKOr<PROGRAM_SPEC, 1>::value;
+    static_assert(GROUPS_PER_BLOCK >= 1 && GROUPS_PER_BLOCK <= 16, "GROUPS_PER_BLOCK must be in [1, 16]");
+    static const size_t WORK_GROUP_SIZE = 64 * GROUPS_PER_BLOCK;
+    // Groups live for the whole launch (specifications may keep group-level
+    // partial results, e.g. LDS tallies, and flush them in finalize).
+    static constexpr bool PERSISTENT = true;
 
     static_assert(STACK_SIZE == 0, "only a flat work stack is supported (as in the reference)");
 
@@ -167,7 +193,29 @@ class AsyncProgram {
         unsigned long long int made;
         unsigned long long int idle;
         uint32_t idle_streak;
+        uint32_t queue_check;  // cycles until the device queues are consulted again
+#ifdef HZ_PHASE_TIMING
+        unsigned long long int phase[8];
+#endif
     };
+
+#ifdef HZ_PHASE_TIMING
+    struct PhaseTimer {
+        GroupContext& g;
+        unsigned int phase;
+        unsigned long long int start;
+        __device__ PhaseTimer(GroupContext& gc, unsigned int p) : g(gc), phase(p), start(clock64()) {}
+        __device__ ~PhaseTimer() {
+            unsigned long long int now = clock64();
+            if (hz::wave::leader()) g.phase[phase] += now - start;
+        }
+    };
+#define HZ_PHASE(p) PhaseTimer _hz_phase_timer_##p(_grp_ctx, hz::p)
+#else
+#define HZ_PHASE(p) \
+    do {            \
+    } while (0)
+#endif
 
     struct DeviceContext {
         typedef ProgramType ParentProgramType;
@@ -618,23 +666,29 @@ class AsyncProgram {
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
         uint32_t words[WORDS];
-        if (lane < cnt) {
+        {
+            HZ_PHASE(PHASE_READ);
+            if (lane < cnt) {
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+                for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+            }
+            hz::wave::sync();
+            if (lane == 0) {
+                g.free_mask |= (1u << L);
+                g.ready_mask &= ~(1u << L);
+                g.executed += cnt;
+            }
+            hz::wave::sync();
         }
-        hz::wave::sync();
-        if (lane == 0) {
-            g.free_mask |= (1u << L);
-            g.ready_mask &= ~(1u << L);
-            g.executed += cnt;
+        {
+            HZ_PHASE(PHASE_EVAL);
+            if (lane < cnt) {
+                PromiseUnionType pu;
+                __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
+                pu.template loose_eval<ProgramType>(*this, op);
+            }
+            hz::wave::sync();
         }
-        hz::wave::sync();
-        if (lane < cnt) {
-            PromiseUnionType pu;
-            __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
-            pu.template loose_eval<ProgramType>(*this, op);
-        }
-        hz::wave::sync();
         drop_holding_if_empty();
     }
 
@@ -646,17 +700,40 @@ class AsyncProgram {
     enum CycleResult { RUN = 0, DONE = 1 };
 
     // One scheduling cycle (reference exec_cycle, program/async.h:2155-2319).
+    // Without a ready link, queued links take priority over making new work
+    // (that bounds the population parked in the arena), but a group that can
+    // still make work consults the device queues (a device-scope load) only on
+    // its first such cycle of a launch and then every QUEUE_CHECK_PERIOD-th.
+    static constexpr uint32_t QUEUE_CHECK_PERIOD = 16;
+
     __device__ CycleResult exec_cycle() {
         GroupContext& g = _grp_ctx;
-        spill_until_free(MARGIN);
+        HZ_PHASE(PHASE_CYCLE);
+        {
+            HZ_PHASE(PHASE_SPILL);
+            spill_until_free(MARGIN);
+        }
 
         uint32_t rm = hz::wave::first(g.ready_mask);
         if (rm == 0) {
-            if (queues_hint()) {
-                pull_links(1);
-                rm = hz::wave::first(g.ready_mask);
+            bool check = !hz::wave::first(g.can_make_work);
+            if (!check) {
+                uint32_t countdown = hz::wave::first(g.queue_check);
+                check = countdown == 0;
+                if (hz::wave::lane() == 0) g.queue_check = check ? QUEUE_CHECK_PERIOD - 1 : countdown - 1;
+                hz::wave::sync();
+            }
+            if (check) {
+                HZ_PHASE(PHASE_PULL);
+                if (queues_hint()) {
+                    pull_links(1);
+                    rm = hz::wave::first(g.ready_mask);
+                    if (hz::wave::lane() == 0) g.queue_check = 0;  // keep draining while links remain
+                    hz::wave::sync();
+                }
             }
             // Make work while no link is full, bounded so open links still run.
+            HZ_PHASE(PHASE_MAKE);
             for (uint32_t tries = 0; rm == 0 && tries < 4 && hz::wave::first(g.can_make_work); ++tries) {
                 uint32_t c = 0;
                 fullest_open(c);
@@ -667,15 +744,18 @@ class AsyncProgram {
         }
 
         uint32_t L = hz::LINK_NONE;
-        if (rm != 0) {
-            L = lowest(rm);
-        } else {
-            uint32_t c = 0;
-            L = fullest_open(c);
-            if (L != hz::LINK_NONE) {
-                uint32_t op = hz::wave::first(g.link_op[L]);
-                if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
-                hz::wave::sync();
+        {
+            HZ_PHASE(PHASE_PICK);
+            if (rm != 0) {
+                L = lowest(rm);
+            } else {
+                uint32_t c = 0;
+                L = fullest_open(c);
+                if (L != hz::LINK_NONE) {
+                    uint32_t op = hz::wave::first(g.link_op[L]);
+                    if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
+                    hz::wave::sync();
+                }
             }
         }
 
@@ -686,6 +766,7 @@ class AsyncProgram {
         }
 
         // Nothing local: look for queued links anywhere, then test for quiescence.
+        HZ_PHASE(PHASE_IDLE);
         if (pull_links(1) > 0) return RUN;
         if (hz::wave::first(g.can_make_work)) return RUN;
         uint32_t streak = hz::wave::first(g.idle_streak);
@@ -720,13 +801,17 @@ class AsyncProgram {
             g.home = blockIdx.x % _dev_ctx.queue_count;
             g.executed = g.spills = g.fills = g.made = g.idle = 0;
             g.idle_streak = 0;
+            g.queue_check = 0;
+#ifdef HZ_PHASE_TIMING
+            for (int i = 0; i < 8; ++i) g.phase[i] = 0;
+#endif
             hz::wave::atomic_add_acqrel(&_dev_ctx.status->makers, 1);
         }
         hz::wave::sync();
     }
 
     __device__ void init_thread() {
-        _thd_ctx.thread_id = blockIdx.x * 64u + hz::wave::lane();
+        _thd_ctx.thread_id = blockIdx.x * blockDim.x + threadIdx.x;
         _thd_ctx.rand_state = _thd_ctx.thread_id;
     }
 
@@ -769,8 +854,11 @@ class AsyncProgram {
             if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
             if (g.made) hz::wave::atomic_add(&st->make_work_calls, g.made);
             if (g.idle) hz::wave::atomic_add(&st->idle_polls, g.idle);
+#ifdef HZ_PHASE_TIMING
+            for (int i = 0; i < 8; ++i) hz::wave::atomic_add(&st->phase_cycles[i], g.phase[i]);
+#endif
             unsigned int order = hz::wave::atomic_add_acqrel(&st->checkout, 1u);
-            if (order == gridDim.x - 1) {
+            if (order == gridDim.x * GROUPS_PER_BLOCK - 1) {
                 unsigned int old = hz::wave::atomic_and(&st->flags, ~EARLY_HALT_FLAG);
                 int units = hz::wave::atomic_load_acquire(&st->units);
                 if ((old & EARLY_HALT_FLAG) == 0 && units == 0) set_flags(COMPLETION_FLAG);
@@ -830,12 +918,14 @@ class AsyncProgram {
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
 
+    // One workgroup = GROUPS_PER_BLOCK independent groups (wavefronts).
     __device__ static void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int cycles) {
-        __shared__ GroupContext gctx;
-        __shared__ GroupState group_state;
+        __shared__ GroupContext gctx[GROUPS_PER_BLOCK];
+        __shared__ GroupState group_state[GROUPS_PER_BLOCK];
+        const unsigned int g = (GROUPS_PER_BLOCK == 1) ? 0u : (threadIdx.x / 64u);
         ThreadContext tctx;
         ThreadState thread_state;
-        ProgramType prog(ctx, gctx, tctx, state, group_state, thread_state);
+        ProgramType prog(ctx, gctx[g], tctx, state, group_state[g], thread_state);
         prog.exec(cycles);
     }
 
@@ -863,6 +953,7 @@ class AsyncProgram {
             st->checkout = 0;
             st->claim = 0;
             st->promises = st->spills = st->fills = st->make_work_calls = st->launches = st->idle_polls = 0;
+            for (int i = 0; i < 8; ++i) st->phase_cycles[i] = 0;
         }
     }
 };

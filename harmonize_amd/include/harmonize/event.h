@@ -56,6 +56,8 @@ class EventProgram {
 
     static const size_t GROUP_SIZE = 64;
     static const size_t WORK_GROUP_SIZE = 64;
+    // Groups live for one generation only.
+    static constexpr bool PERSISTENT = false;
 
     static const unsigned int BAD_FUNC_ID_FLAG = 0x00000001;
     static const unsigned int OVERFLOW_FLAG = 0x00000004;

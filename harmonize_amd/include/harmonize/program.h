@@ -9,7 +9,7 @@
 #include "event.h"
 
 template <typename ProgType>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(ProgType::WORK_GROUP_SIZE)
     _hz_exec_kernel(typename ProgType::DeviceContext ctx, typename ProgType::DeviceState state, unsigned int n) {
     ProgType::kernel_exec(ctx, state, n);
 }
@@ -53,13 +53,22 @@ __host__ void init(typename ProgType::Instance& instance, size_t group_count, hi
     }
 }
 
-// Launches one execution pass of `group_count` groups (64-lane wavefronts).
+// Workgroups needed for `group_count` groups (rounded up to whole workgroups).
+template <typename ProgType>
+__host__ size_t blocks_for(size_t group_count) {
+    const size_t per = ProgType::WORK_GROUP_SIZE / 64;
+    return (group_count + per - 1) / per;
+}
+
+// Launches one execution pass of `group_count` groups (64-lane wavefronts),
+// rounded up to whole workgroups of ProgType::WORK_GROUP_SIZE threads.
 template <typename ProgType>
 __host__ void exec(typename ProgType::Instance& instance, size_t group_count, size_t n, hipStream_t stream = nullptr) {
     if (group_count == 0) group_count = 1;
     auto ctx = instance.to_context();
-    hipLaunchKernelGGL(_hz_exec_kernel<ProgType>, dim3((unsigned)group_count), dim3(64), 0, stream, ctx,
-                       instance.device_state, (unsigned int)(n > 0xFFFFFFFFull ? 0xFFFFFFFFull : n));
+    hipLaunchKernelGGL(_hz_exec_kernel<ProgType>, dim3((unsigned)blocks_for<ProgType>(group_count)),
+                       dim3((unsigned)ProgType::WORK_GROUP_SIZE), 0, stream, ctx, instance.device_state,
+                       (unsigned int)(n > 0xFFFFFFFFull ? 0xFFFFFFFFull : n));
     if constexpr (!hz::IsAsync<ProgType>::value) {
         hipLaunchKernelGGL(_hz_event_flip_kernel<ProgType>, dim3(1), dim3(64), 0, stream, ctx);
     }

@@ -46,6 +46,9 @@ HZ_CONST_OR(FRAME_SIZE)
 HZ_CONST_OR(POOL_SIZE)
 HZ_CONST_OR(STACK_SIZE)
 HZ_CONST_OR(GROUP_SIZE)
+// Groups (wavefronts) launched per workgroup by the async program type; they
+// share the workgroup's __shared__ memory. Not a reference constant.
+HZ_CONST_OR(GROUPS_PER_BLOCK)
 #undef HZ_CONST_OR
 
 }  // namespace spec

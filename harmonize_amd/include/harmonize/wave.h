@@ -116,6 +116,15 @@ __device__ HZ_INLINE T aggregated_fetch_add(T* counter, T amount) {
 }  // namespace wave
 }  // namespace hz
 
+// Index of the executing group (a wavefront's workgroup). Specifications use
+// it to spread contended atomics over replicas of a tally.
+__device__ HZ_INLINE unsigned int hz_group_index() { return blockIdx.x; }
+// Thread index within the workgroup and the workgroup's thread count: the
+// threads that share a specification's __shared__ data, so that loops over
+// such data (zeroing, flushing) can be split among them.
+__device__ HZ_INLINE unsigned int hz_block_thread() { return threadIdx.x; }
+__device__ HZ_INLINE unsigned int hz_block_threads() { return blockDim.x; }
+
 // Platform hooks used by the portable iterator layer.
 template <typename T>
 __device__ HZ_INLINE T hz_fetch_add(T* counter, T amount) {

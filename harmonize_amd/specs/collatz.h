@@ -68,29 +68,67 @@ struct CollatzOdd {
     }
 };
 
+// Stash links per group, and start indices each thread claims from the
+// shared iterator at once (see make_work).
+#ifndef HZ_COLLATZ_STASH
+#define HZ_COLLATZ_STASH 16
+#endif
+#ifndef HZ_COLLATZ_CLAIM
+#define HZ_COLLATZ_CLAIM 8
+#endif
+
+// Start indices a thread has claimed and not yet launched.
+struct CollatzClaim {
+    unsigned int next;
+    unsigned int end;
+};
+
+template <typename PROGRAM>
+__device__ inline void collatz_launch(PROGRAM& prog, unsigned int index) {
+    unsigned long long start = prog.device.starts ? prog.device.starts[index] : (unsigned long long)index;
+    collatz_dispatch(prog, start, 0u, index);
+}
+
 struct CollatzSpec {
     typedef OpUnion<CollatzEven, CollatzOdd> OpSet;
     typedef CollatzState DeviceState;
+    typedef CollatzClaim ThreadState;
 
-    static const size_t STASH_SIZE = 16;
+    static const size_t STASH_SIZE = HZ_COLLATZ_STASH;
     static const size_t FRAME_SIZE = 8;
     static const size_t POOL_SIZE = 8;
 
     template <typename PROGRAM>
-    __device__ static void initialize(PROGRAM) {}
+    __device__ static void initialize(PROGRAM prog) {
+        prog.thread.next = 0u;
+        prog.thread.end = 0u;
+    }
 
+    // Claimed starts not launched yet are queued now, so a pass that ends
+    // early loses none of them.
     template <typename PROGRAM>
-    __device__ static void finalize(PROGRAM) {}
+    __device__ static void finalize(PROGRAM prog) {
+        while (prog.thread.next < prog.thread.end) {
+            collatz_launch(prog, prog.thread.next);
+            prog.thread.next += 1u;
+        }
+    }
 
-    // Claims one start per thread (reference collaz_hrm.cpp:127-144).
+    // Launches one start per thread per call, as the reference's make_work
+    // does (collaz_hrm.cpp:127-144), but claims HZ_COLLATZ_CLAIM starts from
+    // the shared iterator at a time: that iterator is one atomic counter for
+    // the whole device, and same-address atomics serialise across the XCDs.
     template <typename PROGRAM>
     __device__ static bool make_work(PROGRAM prog) {
-        util::iter::Iter<unsigned int> it = prog.device.iterator->leap(1u);
-        unsigned int index;
-        while (it.step(index)) {
-            unsigned long long start =
-                prog.device.starts ? prog.device.starts[index] : (unsigned long long)index;
-            collatz_dispatch(prog, start, 0u, index);
+        if (prog.thread.next >= prog.thread.end) {
+            util::iter::Iter<unsigned int> it = prog.device.iterator->leap(HZ_COLLATZ_CLAIM);
+            prog.thread.next = it.value;
+            prog.thread.end = it.limit;
+        }
+        if (prog.thread.next < prog.thread.end) {
+            collatz_launch(prog, prog.thread.next);
+            prog.thread.next += 1u;
+            return true;
         }
         return !prog.device.iterator->done();
     }

@@ -67,6 +67,36 @@ typedef struct hz_collatz_state {
     void* iterator;
 } hz_collatz_state;
 
+/* Branching photon walk in a slab (FP32 state, FP64 tallies). The tally
+ * holds `replicas` copies of (bins + HZ_PHOTON_EXTRA_SLOTS) doubles: depth
+ * bins of deposited weight, then reflected, transmitted, roulette-lost,
+ * roulette-gained and capped weight. events[photon] = events of its family. */
+#define HZ_PHOTON_EXTRA_SLOTS 5
+typedef struct hz_photon_params {
+    uint32_t photons;
+    uint32_t bins;
+    uint64_t seed;
+    float thickness;
+    float sigma_t;
+    float albedo;
+    float split_above;    /* split when w / target > split_above */
+    float roulette_below; /* roulette when w / target < roulette_below */
+    float region_width;   /* weight-window region depth; target = 2^-region */
+    uint32_t regions;
+    uint32_t max_split;   /* power of two */
+    uint32_t max_events;  /* per lineage, <= 65535 */
+    uint32_t replicas;    /* tally replicas, power of two */
+} hz_photon_params;
+
+/* Device state of the photon programs (layout of PhotonState). */
+typedef struct hz_photon_state {
+    uint32_t photon_count, seed_lo, seed_hi, bins, replicas, max_split, max_events, regions;
+    float thickness, mean_free_path, albedo, bin_scale, region_scale, split_above, roulette_below, pad;
+    double* tally;
+    uint32_t* events;
+    void* iterator;
+} hz_photon_state;
+
 /* ------------------------------------------- per-program entry points
  * For each <suffix> in collatz_event, collatz_async:
  *
@@ -107,6 +137,8 @@ typedef struct hz_collatz_state {
 
 HZ_DECLARE_PROGRAM_ABI(collatz_event)
 HZ_DECLARE_PROGRAM_ABI(collatz_async)
+HZ_DECLARE_PROGRAM_ABI(photon_event)
+HZ_DECLARE_PROGRAM_ABI(photon_async)
 
 /* ------------------------------------------------------ drivers (layer 2) */
 
@@ -132,6 +164,10 @@ typedef struct hz_run_stats {
     uint32_t groups;          /* groups actually launched */
     uint32_t reserved;
     double elapsed_ms;        /* hipEvent time of the exec loop */
+    uint64_t phase_cycles[8]; /* async, runtime built with HZ_PHASE_TIMING:
+                                 shader cycles summed over groups per phase
+                                 (cycle, spill, pull, make_work, pick, read,
+                                 eval, idle); zero otherwise */
 } hz_run_stats;
 
 /* Runs the Collatz program over `count` starts (d_starts may be NULL: start
@@ -144,6 +180,24 @@ int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count, uint32_t*
  * where (x0, x1, x2, x3) = Philox4x32-10(counter = {i, 0, 0, 0},
  * key = {seed & 0xFFFFFFFF, seed >> 32}). 1 <= bits <= 63. */
 int hz_collatz_make_starts(uint64_t seed, uint32_t bits, uint32_t count, uint64_t* d_out);
+
+/* Runs the photon walk for photons [0, params->photons). d_tally must hold
+ * params->replicas * (bins + HZ_PHOTON_EXTRA_SLOTS) doubles and d_events
+ * params->photons counts; both are zeroed first. kind = HZ_EVENT or HZ_ASYNC. */
+int hz_photon_run(int kind, const hz_photon_params* params, double* d_tally, uint32_t* d_events,
+                  const hz_run_config* config, hz_run_stats* stats);
+
+/* Fills a hz_photon_params with the workload's defaults: 1e7 photons,
+ * 1000 bins, thickness 20, sigma_t 1, albedo 0.9, split above 2, roulette
+ * below 0.1, region width 2 (10 regions), max_split 8, max_events 65535,
+ * 256 replicas, seed 0x5EED. */
+void hz_photon_defaults(hz_photon_params* params);
+
+/* Diagnostic: evaluates the specifications' deterministic math on the GPU:
+ * d_log[i] = det_log(d_in[i]), d_sqrt[i] = det_sqrt(d_in[i]) and
+ * d_philox[4i..4i+3] = Philox4x32-10(counter {i, 3i, ~i, 7},
+ * key {0x5EED, i ^ 0xABCD}), so tests can compare them with the host bit for bit. */
+int hz_detmath_eval(const double* d_in, double* d_log, double* d_sqrt, uint32_t* d_philox, uint32_t n);
 
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {

@@ -60,6 +60,8 @@ class HostProgram {
         typedef OpUnionLookup<TYPE, OpSet> type;
     };
 
+    static constexpr bool PERSISTENT = false;
+
     struct Runner {
         std::deque<Entry> queue;
         RunStats stats;
@@ -119,6 +121,7 @@ class HostProgram {
 
     // Runs a specification to completion on one host thread.
     static RunStats run(DeviceState state) {
+        hz_host_group_slot() = 0;
         Runner r;
         GroupState g{};
         ThreadState t{};
@@ -159,6 +162,7 @@ struct OmpProgram {
 
 #pragma omp parallel num_threads(threads)
         {
+            hz_host_group_slot() = omp_get_thread_num();
             typename Host::Runner r;
             GroupState g{};
             ThreadState t{};

@@ -21,6 +21,17 @@
 #define __global__
 #define __forceinline__ inline __attribute__((always_inline))
 
+// The host executors run one group per thread; this is that thread's index.
+inline int& hz_host_group_slot() {
+    static thread_local int slot = 0;
+    return slot;
+}
+inline unsigned int hz_group_index() { return (unsigned int)hz_host_group_slot(); }
+inline unsigned int hz_block_thread() { return 0u; }
+inline unsigned int hz_block_threads() { return 1u; }
+inline void __syncthreads() {}
+#define __shared__ static
+
 template <typename T>
 inline T hz_fetch_add(T* counter, T amount) {
     return __atomic_fetch_add(counter, amount, __ATOMIC_SEQ_CST);

@@ -6,9 +6,11 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "host_program.h"
 #include "collatz.h"
+#include "photon.h"
 #include "philox.h"
 
 extern "C" {
@@ -118,5 +120,69 @@ void oracle_collatz_make_starts(uint64_t seed, uint32_t bits, uint32_t count, ui
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+// Photon-walk parameters, laid out as hz_photon_params in harmonize_amd.h.
+struct oracle_photon_params {
+    uint32_t photons;
+    uint32_t bins;
+    uint64_t seed;
+    float thickness;
+    float sigma_t;
+    float albedo;
+    float split_above;
+    float roulette_below;
+    float region_width;
+    uint32_t regions;
+    uint32_t max_split;
+    uint32_t max_events;
+    uint32_t replicas;
+};
+
+// Runs the photon specification on the host executors. tally receives the
+// replica-summed tally (bins + PHOTON_EXTRA_SLOTS doubles), events one count
+// per photon. Photons [first, first + count) are run (ids stay global).
+int oracle_photon_program(const oracle_photon_params* prm, uint32_t first, uint32_t count, double* tally,
+                          uint32_t* events, int threads, oracle_run_stats* stats) {
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    unsigned int replicas = 1;
+    while (replicas < (unsigned int)used) replicas <<= 1;
+    const unsigned int stride = prm->bins + PHOTON_EXTRA_SLOTS;
+    std::vector<double> rep((size_t)replicas * stride, 0.0);
+    util::iter::AtomicIter<unsigned int> iter(first, first + count);
+    PhotonState st{};
+    st.photon_count = prm->photons;
+    st.seed_lo = (uint32_t)prm->seed;
+    st.seed_hi = (uint32_t)(prm->seed >> 32);
+    st.bins = prm->bins;
+    st.replicas = replicas;
+    st.max_split = prm->max_split;
+    st.max_events = prm->max_events;
+    st.regions = prm->regions;
+    st.thickness = prm->thickness;
+    st.mean_free_path = 1.0f / prm->sigma_t;
+    st.albedo = prm->albedo;
+    st.bin_scale = (float)prm->bins / prm->thickness;
+    st.region_scale = 1.0f / prm->region_width;
+    st.split_above = prm->split_above;
+    st.roulette_below = prm->roulette_below;
+    st.tally = rep.data();
+    st.events = events;
+    st.iterator = &iter;
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s = used == 1 ? oracle::HostProgram<PhotonSpec>::run(st)
+                                   : oracle::OmpProgram<PhotonSpec>::run(st, used);
+    auto t1 = std::chrono::steady_clock::now();
+    for (unsigned int i = 0; i < stride; ++i) {
+        double sum = 0.0;
+        for (unsigned int k = 0; k < replicas; ++k) sum += rep[(size_t)k * stride + i];
+        tally[i] = sum;
+    }
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Deterministic math of the photon specification, exposed for tests.
+double oracle_det_log(double x) { return hz::det::det_log(x); }
+double oracle_det_sqrt(double x) { return hz::det::det_sqrt(x); }
 
 }  // extern "C"

@@ -85,3 +85,42 @@ def philox(counter, key):
     o = np.zeros(4, dtype=np.uint32)
     lib().oracle_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
     return [int(x) for x in o]
+
+
+class PhotonParams(ctypes.Structure):
+    _fields_ = [
+        ("photons", ctypes.c_uint32), ("bins", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("thickness", ctypes.c_float), ("sigma_t", ctypes.c_float), ("albedo", ctypes.c_float),
+        ("split_above", ctypes.c_float), ("roulette_below", ctypes.c_float), ("region_width", ctypes.c_float),
+        ("regions", ctypes.c_uint32), ("max_split", ctypes.c_uint32), ("max_events", ctypes.c_uint32),
+        ("replicas", ctypes.c_uint32),
+    ]
+
+
+PHOTON_DEFAULTS = dict(photons=10_000_000, bins=1000, seed=0x5EED, thickness=20.0, sigma_t=1.0, albedo=0.9,
+                       split_above=2.0, roulette_below=0.1, region_width=2.0, regions=10, max_split=8,
+                       max_events=65535, replicas=256)
+
+_SIGS.update({
+    "oracle_photon_program": (ctypes.c_int, [ctypes.POINTER(PhotonParams), ctypes.c_uint32, ctypes.c_uint32, _VP,
+                                             _VP, ctypes.c_int, ctypes.POINTER(OracleStats)]),
+    "oracle_det_log": (ctypes.c_double, [ctypes.c_double]),
+    "oracle_det_sqrt": (ctypes.c_double, [ctypes.c_double]),
+})
+
+
+def photon_params(**overrides):
+    d = dict(PHOTON_DEFAULTS)
+    d.update(overrides)
+    return PhotonParams(**d)
+
+
+def photon_program(params, first=0, count=None, threads=1):
+    """Runs photons [first, first+count) on the host executors; returns (tally, events[photons], stats)."""
+    count = params.photons - first if count is None else count
+    tally = np.zeros(params.bins + 5)
+    events = np.zeros(params.photons, dtype=np.uint32)
+    st = OracleStats()
+    lib().oracle_photon_program(ctypes.byref(params), first, count, tally.ctypes.data, events.ctypes.data, threads,
+                                ctypes.byref(st))
+    return tally, events, st

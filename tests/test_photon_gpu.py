@@ -1,0 +1,116 @@
+"""GPU parity of the photon walk against the oracle: per-photon event counts
+bit-exact, FP64 tallies within 1e-12 relative (their accumulation order
+differs), through the C ABI."""
+
+import numpy as np
+import pytest
+
+import harmonize_amd as hz
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def assert_tally_close(got, ref):
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    scale = np.maximum(np.abs(got), np.abs(ref))
+    bad = np.abs(got - ref) > RTOL * scale
+    assert not bad.any(), (np.nonzero(bad)[0][:10], got[bad][:5], ref[bad][:5])
+
+
+def balance(tally, n):
+    absorbed = tally[:-5].sum()
+    refl, trans, lost, gained, capped = tally[-5:]
+    return absorbed + refl + trans + lost - gained + capped - n
+
+
+def test_detmath_bit_identical_on_gpu():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(2.0 ** -24, 1.0, 200000), 2.0 ** rng.uniform(-900, 900, 50000),
+                        np.array([1.0, 0.5, 2.0, 2.0 ** -24])])
+    n = x.size
+    din = hz.DeviceArray.from_numpy(x)
+    dlog, dsqrt = hz.DeviceArray(n, np.float64), hz.DeviceArray(n, np.float64)
+    dph = hz.DeviceArray(4 * n, np.uint32)
+    hz.check(hz.lib().hz_detmath_eval(din.ptr, dlog.ptr, dsqrt.ptr, dph.ptr, n), "hz_detmath_eval")
+    glog, gsqrt, gph = dlog.numpy(), dsqrt.numpy(), dph.numpy().reshape(n, 4)
+    L = O.lib()
+    idx = np.arange(0, n, 7)
+    for i in idx:
+        v = float(x[i])
+        assert glog[i] == L.oracle_det_log(v), (i, v)
+        assert gsqrt[i] == L.oracle_det_sqrt(v), (i, v)
+    for i in idx[:3000]:
+        i = int(i)
+        ref = O.philox([i, (3 * i) & 0xFFFFFFFF, (~i) & 0xFFFFFFFF, 7], [0x5EED, i ^ 0xABCD])
+        assert list(gph[i]) == ref
+
+
+@pytest.mark.parametrize("kind", ["async", "event"])
+def test_small_run_matches_sequential_oracle(kind):
+    n = 50000
+    p_gpu = hz.photon_params(photons=n)
+    tally, events, st = hz.photon_run(kind, p_gpu, max_launches=100000)
+    ref_t, ref_e, ref_st = O.photon_program(O.photon_params(photons=n), threads=1)
+    assert np.array_equal(events, ref_e)
+    assert st["promises"] == ref_st.promises
+    assert_tally_close(tally, ref_t)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000])
+def test_ragged_photon_counts(n):
+    ref_t, ref_e, _ = O.photon_program(O.photon_params(photons=n), threads=1)
+    for kind in ("async", "event"):
+        tally, events, _ = hz.photon_run(kind, hz.photon_params(photons=n), max_launches=100000)
+        assert np.array_equal(events, ref_e)
+        assert_tally_close(tally, ref_t)
+
+
+def test_albedo_zero_gpu():
+    n = 100000
+    tally, events, st = hz.photon_run("async", hz.photon_params(photons=n, albedo=0.0))
+    assert (events == 1).all() and st["promises"] == n
+
+
+@pytest.fixture(scope="module")
+def full_oracle():
+    p = O.photon_params()
+    return O.photon_program(p, threads=0)
+
+
+@pytest.mark.parametrize("kind", ["async", "event"])
+def test_full_scale_1e7_photons(full_oracle, kind):
+    ref_t, ref_e, ref_st = full_oracle
+    p = hz.photon_params()
+    tally, events, st = hz.photon_run(kind, p, max_launches=1 << 20)
+    assert p.photons == 10_000_000
+    assert np.array_equal(events, ref_e)
+    assert st["promises"] == ref_st.promises == int(events.sum(dtype=np.uint64))
+    assert_tally_close(tally, ref_t)
+    assert abs(balance(tally, p.photons)) < 1e-9 * p.photons
+
+
+def test_async_relaunch_and_spill_path():
+    n = 200000
+    ref_t, ref_e, _ = O.photon_program(O.photon_params(photons=n), threads=0)
+    tally, events, st = hz.photon_run("async", hz.photon_params(photons=n), cycles=40, groups=512,
+                                      max_launches=100000)
+    assert st["launches"] > 1 and st["spills"] > 0
+    assert np.array_equal(events, ref_e)
+    assert_tally_close(tally, ref_t)
+
+
+def test_branching_heavy_variant():
+    # Weak absorption and a steep window: families of hundreds of events with
+    # repeated 2-8 way splits.
+    n = 20000
+    kw = dict(photons=n, albedo=0.97, max_split=8, region_width=1.5, regions=13)
+    ref_t, ref_e, ref_st = O.photon_program(O.photon_params(**kw), threads=0)
+    for kind in ("async", "event"):
+        tally, events, st = hz.photon_run(kind, hz.photon_params(**kw), max_launches=1 << 20)
+        assert np.array_equal(events, ref_e)
+        assert st["promises"] == ref_st.promises
+        assert_tally_close(tally, ref_t)
