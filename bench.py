@@ -43,6 +43,20 @@ def algorithmic_bytes(promises: int, starts: int) -> int:
     return 2 * PROMISE_BYTES * promises + starts * (8 + 4)
 
 
+# HBM traffic of one AsyncProgram exec launch on this workload, from two
+# `rocprofv3 --pmc` passes over `bench.py --leg collatz_async` (FETCH_SIZE,
+# then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter
+# reads for a 1 GiB device-to-device copy (`bench.py --leg copy`): FETCH_SIZE
+# reads 512 MiB for the copy's 1 GiB of reads (factor 2.0) and WRITE_SIZE
+# reads 1 GiB for its 1 GiB of writes (factor 1.0).
+TRAFFIC_KIB = {"fetch": 342551.0, "write": 1130255.8}
+TRAFFIC_FACTOR = {"fetch": 2.0, "write": 1.0}
+
+
+def measured_traffic_bytes() -> int:
+    return int(sum(TRAFFIC_KIB[k] * TRAFFIC_FACTOR[k] * 1024 for k in TRAFFIC_KIB))
+
+
 def cpu_baseline(threads_all: int) -> dict:
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the oracle, used only as the CPU baseline leg
@@ -71,6 +85,68 @@ def cpu_baseline(threads_all: int) -> dict:
     }
 
 
+def photon_comparison(hz, repeats: int = 3) -> dict:
+    """Branching photon walk, 1e7 photons: AsyncProgram vs EventProgram, same spec,
+    each the best of `repeats` runs at its best measured launch shape."""
+    p = hz.photon_params()
+    tally = hz.DeviceArray(p.replicas * (p.bins + hz.PHOTON_EXTRA_SLOTS), np.float64)
+    events = hz.DeviceArray(p.photons, np.uint32)
+
+    def best(kind, **cfg):
+        runs = [hz.photon_run(kind, p, tally, events, **cfg)[2] for _ in range(repeats)]
+        return min(runs, key=lambda r: r["elapsed_ms"])
+
+    a = best("async")
+    e = best("event", groups=2048, cycles=8, max_launches=1 << 20)
+    if a["promises"] != e["promises"]:
+        raise RuntimeError("photon: async and event evaluated different numbers of events")
+    rate = lambda r: r["promises"] / (r["elapsed_ms"] * 1e-3)
+    return {
+        "workload": "branching photon walk: 1e7 photons, slab 20, sigma_t 1, albedo 0.9, split > 2, "
+                    "roulette < 0.1, Philox4x32-10, 1000 FP64 depth bins",
+        "events": a["promises"],
+        "async": {"value": rate(a), "unit": "events/s", "ms": a["elapsed_ms"], "groups": a["groups"]},
+        "event": {"value": rate(e), "unit": "events/s", "ms": e["elapsed_ms"], "groups": e["groups"],
+                  "launches": e["launches"]},
+        "async_vs_event": rate(a) / rate(e),
+    }
+
+
+def _leg_collatz(kind):
+    def run(hz):
+        starts = hz.collatz_make_starts(SEED, BITS, COUNT)
+        steps_out = hz.DeviceArray(COUNT, np.uint32)
+        cfg = dict(arena=ARENA_PROMISES // LINK_PROMISES) if kind == "async" else dict(max_launches=1 << 20)
+        return hz.collatz_run(kind, steps_out, starts, COUNT, **cfg)
+    return run
+
+
+def _leg_photon(kind):
+    def run(hz):
+        cfg = {} if kind == "async" else dict(groups=2048, cycles=8)
+        return hz.photon_run(kind, hz.photon_params(), max_launches=1 << 20, **cfg)[2]
+    return run
+
+
+def _leg_copy(hz):
+    """1 GiB device-to-device copy: the calibration of the HBM byte counters."""
+    nbytes = 1 << 30
+    a = hz.DeviceArray(nbytes // 4, np.uint32)
+    b = hz.DeviceArray(nbytes // 4, np.uint32)
+    hz.device_copy(b, a, nbytes)
+    hz.check(hz.lib().hz_device_synchronize(), "sync")
+    return {"bytes": nbytes}
+
+
+LEGS = {
+    "collatz_async": _leg_collatz("async"),
+    "collatz_event": _leg_collatz("event"),
+    "photon_async": _leg_photon("async"),
+    "photon_event": _leg_photon("event"),
+    "copy": _leg_copy,
+}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,14 +155,20 @@ def main() -> int:
     ap.add_argument("--groups", type=int, default=0, help="AsyncProgram groups (0 = fill the device)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-event", action="store_true", help="skip the EventProgram comparator")
+    ap.add_argument("--no-photon", action="store_true", help="skip the photon-walk Async vs Event comparison")
+    ap.add_argument("--leg", choices=sorted(LEGS), help="run one leg once (for counter collection) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this runtime is measured on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this runtime is measured on one GPU; --gpus must be 1 (sharding a program "
+                                   "across GPUs is not built)"}))
         return 2
 
     import harmonize_amd as hz
+
+    if args.leg:
+        print(json.dumps({"leg": args.leg, **LEGS[args.leg](hz)}), flush=True)
+        return 0
 
     facts = hz.device_facts()
     starts = hz.collatz_make_starts(SEED, BITS, COUNT)
@@ -146,7 +228,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": measured_traffic_bytes(),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
             "kernel": "_hz_exec_kernel<AsyncProgram<CollatzSpec>>",
             "kernel_ms": per_launch_ms,
             "bytes_rule": "2 x 16 B per promise (enqueue + dequeue of the packed payload) + 12 B per start",
@@ -163,6 +246,9 @@ def main() -> int:
             "groups": ev["groups"],
         }
         out["async_vs_event"] = value / out["event"]["value"]
+
+    if not args.no_photon:
+        out["photon"] = photon_comparison(hz)
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
     out["plain_kernel"] = {

@@ -87,6 +87,7 @@ _SIGNATURES = {
     "harmonize_memcpy_host_to_device": (None, [_VP, _VP, ctypes.c_size_t]),
     "harmonize_memcpy_device_to_host": (None, [_VP, _VP, ctypes.c_size_t]),
     "hz_device_synchronize": (ctypes.c_int, []),
+    "hz_device_copy": (ctypes.c_int, [_VP, _VP, ctypes.c_size_t]),
     "hz_device_query": (ctypes.c_int, [ctypes.POINTER(DeviceFacts)]),
     "hz_collatz_run": (
         ctypes.c_int,
@@ -194,6 +195,13 @@ class DeviceArray:
             pass
 
 
+def device_copy(dst: "DeviceArray", src: "DeviceArray", nbytes: int) -> None:
+    """Synchronous device-to-device copy of nbytes between two device arrays."""
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("device_copy: nbytes exceeds an array")
+    check(lib().hz_device_copy(dst.ptr, src.ptr, nbytes), "hz_device_copy")
+
+
 def device_facts() -> dict:
     f = DeviceFacts()
     check(lib().hz_device_query(ctypes.byref(f)), "hz_device_query")
@@ -267,8 +275,41 @@ _SIGNATURES.update(
         ),
         "hz_photon_defaults": (None, [ctypes.POINTER(PhotonParams)]),
         "hz_detmath_eval": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_uint32]),
+        "hz_barrier_run": (
+            ctypes.c_int,
+            [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.POINTER(RunConfig),
+             ctypes.POINTER(RunStats)],
+        ),
     }
 )
+for _s in ("barrier_async",):
+    _SIGNATURES.update(
+        {
+            f"alloc_state_{_s}": (_VP, []),
+            f"free_state_{_s}": (None, [_VP]),
+            f"load_state_device_{_s}": (None, [_VP, _VP]),
+            f"store_state_device_{_s}": (None, [_VP, _VP]),
+            f"alloc_program_{_s}": (_VP, [_VP, ctypes.c_size_t]),
+            f"free_program_{_s}": (None, [_VP]),
+            f"init_program_{_s}": (None, [_VP, ctypes.c_size_t]),
+            f"exec_program_{_s}": (None, [_VP, ctypes.c_size_t, ctypes.c_size_t]),
+            f"complete_{_s}": (ctypes.c_int, [_VP]),
+            f"clear_flags_{_s}": (None, [_VP]),
+            f"set_device_{_s}": (None, [ctypes.c_int]),
+        }
+    )
+
+
+def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
+    """Runs the remapping-barrier program; returns (counter, hits[dep], violations, stats)."""
+    counter = DeviceArray(1, np.uint32)
+    violations = DeviceArray(1, np.uint32)
+    hits = DeviceArray(max(1, dep_count), np.uint32)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(lib().hz_barrier_run(dep_count, non_dep_count, barrier_count, counter.ptr, hits.ptr, violations.ptr,
+                               ctypes.byref(config), ctypes.byref(st)), "hz_barrier_run")
+    return int(counter.numpy()[0]), hits.numpy()[:dep_count], int(violations.numpy()[0]), st.as_dict()
 for _s in ("photon_event", "photon_async"):
     _SIGNATURES.update(
         {

@@ -64,3 +64,10 @@ extern "C" void harmonize_memcpy_device_to_host(void* host_ptr, void* dev_ptr, s
 extern "C" int hz_device_synchronize(void) {
     return guarded([&] { check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
 }
+
+extern "C" int hz_device_copy(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        if (bytes && (!dst || !src)) throw std::invalid_argument("hz_device_copy: null pointer");
+        check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+    });
+}

@@ -12,4 +12,5 @@
 #include "harmonize/host.h"
 #include "harmonize/cli.h"
 #include "harmonize/program.h"
+#include "harmonize/barrier.h"
 

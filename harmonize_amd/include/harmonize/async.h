@@ -226,6 +226,7 @@ class AsyncProgram {
         unsigned int* free_slots;   // [queue_capacity]
         unsigned int* arena;        // [links][WORDS][64]
         unsigned int* link_meta;    // [links]: op | count << 16
+        unsigned int* link_aux;     // [links]: barrier bookkeeping (next link, fill)
         unsigned int links;
         unsigned int queue_count;
         unsigned int queue_mask;    // queue_capacity - 1
@@ -247,6 +248,7 @@ class AsyncProgram {
         util::host::DevBuf<unsigned int> free_slots;
         util::host::DevBuf<unsigned int> arena;
         util::host::DevBuf<unsigned int> link_meta;
+        util::host::DevBuf<unsigned int> link_aux;
         DeviceState device_state;
         hz::AsyncStatus host_status{};
 
@@ -274,6 +276,7 @@ class AsyncProgram {
               free_slots((size_t)pow2_at_least(arena_links < 1 ? 1 : arena_links)),
               arena((size_t)(arena_links < 1 ? 1 : arena_links) * WORDS * 64u),
               link_meta((size_t)(arena_links < 1 ? 1 : arena_links)),
+              link_aux((size_t)2 * (arena_links < 1 ? 1 : arena_links)),
               device_state(gs) {
             if (arena_links > 0x7FFFFFFFull) throw std::runtime_error("AsyncProgram::Instance: arena too large");
             reset(nullptr);
@@ -289,6 +292,7 @@ class AsyncProgram {
             c.free_slots = free_slots.data();
             c.arena = arena.data();
             c.link_meta = link_meta.data();
+            c.link_aux = link_aux.data();
             c.links = arena_size;
             c.queue_count = queue_count;
             c.queue_mask = queue_capacity - 1u;
@@ -910,6 +914,164 @@ class AsyncProgram {
         immediate_call_cast<TYPE>(promise);
     }
 
+    // Queues a promise held as a PromiseEnum (UnitBarrier release).
+    __device__ void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
+
+    // ------------------------------------------------------ remapping barrier
+    // (protocol described in barrier.h). link_aux[2L] = next link in a
+    // barrier's full list; link_aux[2L+1] = (target << 16) | filled.
+
+    template <typename BARRIER, typename OP_TYPE>
+    __device__ void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
+        constexpr OpDisc D = Lookup<OP_TYPE>::type::DISC;
+        if (hz::wave::atomic_load(&b.semaphore.sem) == 0u) {
+            async_call_cast<OP_TYPE>(0, promise);
+            return;
+        }
+        PromiseUnionType pu;
+        pu.template cast<OP_TYPE>() = promise;
+        uint32_t words[WORDS];
+        __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
+        hz::wave::atomic_add(&b.count, 1u);
+        unsigned long long* part = &b.partial_table[D];
+        for (uint32_t attempt = 0; attempt < SLOT_SPIN_LIMIT; ++attempt) {
+            unsigned long long old = hz::wave::atomic_add_acqrel(part, 1ull);
+            unsigned int L = (unsigned int)(old >> 32);
+            unsigned int s = (unsigned int)old;
+            if (L == hz::BARRIER_RELEASED) {
+                async_call_cast<OP_TYPE>(0, promise);
+                return;
+            }
+            if (L != hz::BARRIER_NO_LINK && s < 64u) {
+                barrier_write(b, L, s, D, words);
+                return;
+            }
+            if ((L == hz::BARRIER_NO_LINK && s == 0u) || (L != hz::BARRIER_NO_LINK && s == 64u)) {
+                // This lane installs a fresh link with its promise in slot 0.
+                unsigned int fresh = arena_alloc_any();
+                if (fresh == hz::LINK_NONE) {
+                    set_flags(ARENA_EXHAUSTED_FLAG);
+                    return;
+                }
+                hz::wave::atomic_store(&_dev_ctx.link_aux[2u * fresh + 1u], 64u << 16);
+                write_arena_slot(fresh, 0u, words);
+                // The count word must be set before a lane that reads the
+                // installed link from the partial word counts into it.
+                hz::wave::fence_release();
+                unsigned long long want = ((unsigned long long)fresh << 32) | 1ull;
+                unsigned long long cur = hz::wave::atomic_load(part);
+                while (true) {
+                    unsigned int cl = (unsigned int)(cur >> 32);
+                    if (cl == hz::BARRIER_RELEASED) {  // released meanwhile: run it now
+                        queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh);
+                        async_call_cast<OP_TYPE>(0, promise);
+                        return;
+                    }
+                    unsigned long long seen = atomicCAS(part, cur, want);
+                    if (seen == cur) break;
+                    cur = seen;
+                }
+                barrier_finish(b, fresh, D);
+                return;
+            }
+            // Another lane is installing a link: back off and retry.
+            hz::wave::sleep_short();
+        }
+        set_flags(QUEUE_STALL_FLAG);
+    }
+
+    template <typename BARRIER>
+    __device__ void barrier_release(BARRIER& b) {
+        constexpr unsigned int NOPS = BARRIER::TYPE_COUNT;
+        for (unsigned int d = 0; d < NOPS; ++d) {
+            unsigned long long old =
+                atomicExch(&b.partial_table[d], (unsigned long long)hz::BARRIER_RELEASED << 32);
+            unsigned int L = (unsigned int)(old >> 32);
+            unsigned int s = (unsigned int)old;
+            if (L >= _dev_ctx.links || s >= 64u) continue;  // none, or full: its writers hand it on
+            // Close the partial link at its reserved count.
+            unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], (s - 64u) << 16);
+            if ((before & 0xFFFFu) == s) barrier_complete(b, L, d, s);
+        }
+        unsigned int h = atomicExch(&b.full_head, hz::BARRIER_RELEASED);
+        hz::wave::fence_acquire();
+        for (uint32_t guard = 0; h < _dev_ctx.links && guard < _dev_ctx.links; ++guard) {
+            unsigned int next = hz::wave::atomic_load(&_dev_ctx.link_aux[2u * h]);
+            add_units(1);
+            queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), h);
+            h = next;
+        }
+    }
+
+   protected:
+    template <unsigned int I>
+    __device__ void async_enum(PromiseEnum<OpSet>& pe) {
+        if constexpr (I < OPS) {
+            using Op = typename hz::OpOf<I, OpSet>::type;
+            if (pe.disc == I) {
+                async_call_cast<Op>(0, pe.data.template cast<Op>());
+            } else {
+                async_enum<I + 1>(pe);
+            }
+        }
+    }
+
+    // Any-lane arena allocation (the wave's spare cache is single-lane).
+    __device__ unsigned int arena_alloc_any() {
+        unsigned int id = queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
+        if (id != hz::LINK_NONE) return id;
+        if (hz::wave::atomic_load(&_dev_ctx.status->claim) < _dev_ctx.links) {
+            unsigned int claim = hz::wave::atomic_add(&_dev_ctx.status->claim, 1u);
+            if (claim < _dev_ctx.links) return claim;
+        }
+        return queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
+    }
+
+    __device__ void write_arena_slot(unsigned int L, unsigned int slot, const uint32_t* words) {
+        unsigned int* dst = arena_link(L);
+#pragma unroll
+        for (uint32_t w = 0; w < WORDS; ++w) dst[w * 64u + slot] = words[w];
+    }
+
+    template <typename BARRIER>
+    __device__ void barrier_write(BARRIER& b, unsigned int L, unsigned int slot, OpDisc d, const uint32_t* words) {
+        write_arena_slot(L, slot, words);
+        barrier_finish(b, L, d);
+    }
+
+    // Counts one written slot; the lane that completes the link hands it on.
+    template <typename BARRIER>
+    __device__ void barrier_finish(BARRIER& b, unsigned int L, OpDisc d) {
+        hz::wave::fence_release();
+        unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], 1u);
+        unsigned int filled = (before & 0xFFFFu) + 1u;
+        unsigned int target = before >> 16;
+        if (filled == target) barrier_complete(b, L, d, target);
+    }
+
+    // A completed link goes onto the barrier's full list, or to the work
+    // queues once the barrier is released.
+    template <typename BARRIER>
+    __device__ void barrier_complete(BARRIER& b, unsigned int L, OpDisc d, unsigned int count) {
+        _dev_ctx.link_meta[L] = d | (count << 16);
+        hz::wave::fence_release();
+        unsigned int h = hz::wave::atomic_load(&b.full_head);
+        for (uint32_t spins = 0; spins < SLOT_SPIN_LIMIT; ++spins) {
+            if (h == hz::BARRIER_RELEASED) {
+                add_units(1);
+                queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), L);
+                return;
+            }
+            hz::wave::atomic_store(&_dev_ctx.link_aux[2u * L], h);
+            hz::wave::fence_release();
+            unsigned int seen = atomicCAS(&b.full_head, h, L);
+            if (seen == h) return;
+            h = seen;
+        }
+        set_flags(QUEUE_STALL_FLAG);
+    }
+
+   public:
     // The async program has no queue fill fraction (reference async.h:2900-2904).
     template <typename TYPE>
     __device__ float load_fraction() {

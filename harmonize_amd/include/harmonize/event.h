@@ -405,6 +405,35 @@ class EventProgram {
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
 
+    // Queues a promise held as a PromiseEnum (UnitBarrier release).
+    __device__ void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
+
+    // A RemappingBarrier needs the async program's link arena, as in the
+    // reference (sync/barrier.h uses AsyncProgram internals throughout).
+    template <typename BARRIER, typename OP_TYPE>
+    __device__ void barrier_await(BARRIER&, Promise<OP_TYPE>) {
+        static_assert(sizeof(OP_TYPE) == 0, "RemappingBarrier is supported by AsyncProgram, not EventProgram");
+    }
+    template <typename BARRIER>
+    __device__ void barrier_release(BARRIER&) {
+        static_assert(sizeof(BARRIER) == 0, "RemappingBarrier is supported by AsyncProgram, not EventProgram");
+    }
+
+   protected:
+    template <unsigned int I>
+    __device__ void async_enum(PromiseEnum<OpSet>& pe) {
+        if constexpr (I < OPS) {
+            using Op = typename hz::OpOf<I, OpSet>::type;
+            if (pe.disc == I) {
+                async_call_cast<Op>(0, pe.data.template cast<Op>());
+            } else {
+                async_enum<I + 1>(pe);
+            }
+        }
+    }
+
+   public:
+
     __device__ static void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int chunk) {
         __shared__ GroupContext gctx;
         __shared__ GroupState group_state;

@@ -41,6 +41,13 @@ template <typename TYPE, typename BARRIER>
 struct Future;
 
 namespace hz {
+constexpr unsigned int BARRIER_NO_LINK = 0xFFFFFFFFu;   // no link / empty list
+constexpr unsigned int BARRIER_RELEASED = 0xFFFFFFFEu;  // partial word or list head after release
+constexpr unsigned int BARRIER_TAG = 1u;
+constexpr unsigned int UNIT_BARRIER_TAG = 0u;
+}  // namespace hz
+
+namespace hz {
 namespace detail {
 
 template <typename Q, typename... OPS>
@@ -73,6 +80,14 @@ struct OpAt<I, H, T...> {
 };
 
 }  // namespace detail
+
+// The operation with discriminant I in an OpUnion.
+template <unsigned I, typename OP_SET>
+struct OpOf;
+template <unsigned I, typename... OPS>
+struct OpOf<I, OpUnion<OPS...>> {
+    using type = typename detail::OpAt<I, OPS...>::type;
+};
 }  // namespace hz
 
 // Compile-time facts about one operation type relative to an operation set.

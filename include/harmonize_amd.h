@@ -97,8 +97,23 @@ typedef struct hz_photon_state {
     void* iterator;
 } hz_photon_state;
 
+/* Remapping-barrier program (after the reference's barrier demo,
+ * examples/cpp/barrier_demo/demo.cpp:26-44): Step(id) for ids
+ * [0, dep + non_dep); a dependent id awaits barrier[id % barriers] with a
+ * held Step and decrements that barrier's semaphore. Layout of
+ * BarrierDemoState. */
+typedef struct hz_barrier_state {
+    uint32_t dep_count, non_dep_count, barrier_count, pad;
+    void* source_id_iter;
+    uint32_t* counter;
+    uint32_t* hits;
+    uint32_t* violations;
+    void* barrier;
+} hz_barrier_state;
+
 /* ------------------------------------------- per-program entry points
- * For each <suffix> in collatz_event, collatz_async:
+ * For each <suffix> in collatz_event, collatz_async, photon_event,
+ * photon_async, barrier_async (the barrier needs AsyncProgram):
  *
  *   void* alloc_state_<suffix>(void);
  *       Device allocation sized for the program's device state
@@ -139,6 +154,7 @@ HZ_DECLARE_PROGRAM_ABI(collatz_event)
 HZ_DECLARE_PROGRAM_ABI(collatz_async)
 HZ_DECLARE_PROGRAM_ABI(photon_event)
 HZ_DECLARE_PROGRAM_ABI(photon_async)
+HZ_DECLARE_PROGRAM_ABI(barrier_async)
 
 /* ------------------------------------------------------ drivers (layer 2) */
 
@@ -199,6 +215,13 @@ void hz_photon_defaults(hz_photon_params* params);
  * key {0x5EED, i ^ 0xABCD}), so tests can compare them with the host bit for bit. */
 int hz_detmath_eval(const double* d_in, double* d_log, double* d_sqrt, uint32_t* d_philox, uint32_t n);
 
+/* Runs the barrier program (AsyncProgram only, as in the reference). Writes
+ * *d_counter = Step evaluations (expected 2 * dep + non_dep),
+ * d_hits[i] = runs of the step held for dependent i (expected 1), and
+ * *d_violations = held steps that ran before their barrier's release. */
+int hz_barrier_run(uint32_t dep_count, uint32_t non_dep_count, uint32_t barrier_count, uint32_t* d_counter,
+                   uint32_t* d_hits, uint32_t* d_violations, const hz_run_config* config, hz_run_stats* stats);
+
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {
     int32_t device;
@@ -212,6 +235,10 @@ typedef struct hz_device_facts {
 int hz_device_query(hz_device_facts* facts);
 
 int hz_device_synchronize(void);
+
+/* Device-to-device copy of `bytes` bytes, synchronous (used to calibrate
+ * HBM byte counters against a copy of known size). */
+int hz_device_copy(void* dst, const void* src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -34,6 +35,7 @@
 #include "harmonize/promise.h"
 #include "harmonize/iter.h"
 #include "harmonize/spec.h"
+#include "harmonize/barrier.h"
 
 namespace oracle {
 
@@ -108,6 +110,55 @@ class HostProgram {
     // A GPU pass halted early is resumed by the host loop; the sequential
     // executor therefore just records the request and carries on.
     void halt_early() { _run.stats.halts++; }
+
+    void async_call(PromiseEnum<OpSet> pe) {
+        _run.queue.emplace_back(pe.disc, pe.data);
+        if (_run.queue.size() > _run.stats.peak_queue) _run.stats.peak_queue = _run.queue.size();
+    }
+
+    // RemappingBarrier (reference sync/barrier.h:460-639): promises awaited
+    // on an unreleased barrier wait in a side table shared by all host
+    // threads; release moves them to the releasing thread's queue. The
+    // semaphore test and the insertion happen under the table's lock, and a
+    // release zeroes the semaphore before it takes the lock, so no promise
+    // can be inserted after its barrier was drained.
+    static std::mutex& barrier_lock() {
+        static std::mutex m;
+        return m;
+    }
+    static std::unordered_map<const void*, std::vector<Entry>>& barrier_table() {
+        static std::unordered_map<const void*, std::vector<Entry>> t;
+        return t;
+    }
+
+    template <typename BARRIER, typename OP_TYPE>
+    void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
+        {
+            std::lock_guard<std::mutex> guard(barrier_lock());
+            if (__atomic_load_n(&b.semaphore.sem, __ATOMIC_SEQ_CST) != 0u) {
+                PromiseUnionType pu;
+                pu.template cast<OP_TYPE>() = promise;
+                barrier_table()[&b].emplace_back(Lookup<OP_TYPE>::type::DISC, pu);
+                return;
+            }
+        }
+        async_call<OP_TYPE>(promise);
+    }
+
+    template <typename BARRIER>
+    void barrier_release(BARRIER& b) {
+        std::vector<Entry> held;
+        {
+            std::lock_guard<std::mutex> guard(barrier_lock());
+            auto it = barrier_table().find(&b);
+            if (it != barrier_table().end()) {
+                held.swap(it->second);
+                barrier_table().erase(it);
+            }
+        }
+        for (auto& e : held) _run.queue.push_back(e);
+        if (_run.queue.size() > _run.stats.peak_queue) _run.stats.peak_queue = _run.queue.size();
+    }
 
     // Evaluates one queued promise; false if the queue is empty.
     bool step() {

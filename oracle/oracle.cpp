@@ -11,6 +11,7 @@
 #include "host_program.h"
 #include "collatz.h"
 #include "photon.h"
+#include "barrier_demo.h"
 #include "philox.h"
 
 extern "C" {
@@ -177,6 +178,37 @@ int oracle_photon_program(const oracle_photon_params* prm, uint32_t first, uint3
         for (unsigned int k = 0; k < replicas; ++k) sum += rep[(size_t)k * stride + i];
         tally[i] = sum;
     }
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Runs the barrier specification on the host executors (same outputs as
+// hz_barrier_run).
+int oracle_barrier_program(uint32_t dep, uint32_t non_dep, uint32_t barrier_count, uint32_t* counter, uint32_t* hits,
+                           uint32_t* violations, int threads, oracle_run_stats* stats) {
+    using Barrier = RemappingBarrier<OpUnion<BarrierStep>>;
+    std::vector<Barrier> barriers(barrier_count);
+    for (uint32_t i = 0; i < barrier_count; ++i) {
+        barriers[i] = Barrier::blank(dep / barrier_count + (i < dep % barrier_count ? 1u : 0u));
+    }
+    util::iter::AtomicIter<unsigned int> iter(0u, dep + non_dep);
+    *counter = 0;
+    *violations = 0;
+    for (uint32_t i = 0; i < dep; ++i) hits[i] = 0;
+    BarrierDemoState st{};
+    st.dep_count = dep;
+    st.non_dep_count = non_dep;
+    st.barrier_count = barrier_count;
+    st.source_id_iter = &iter;
+    st.counter = counter;
+    st.hits = hits;
+    st.violations = violations;
+    st.barrier = barriers.data();
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s = used == 1 ? oracle::HostProgram<BarrierDemoSpec>::run(st)
+                                   : oracle::OmpProgram<BarrierDemoSpec>::run(st, used);
+    auto t1 = std::chrono::steady_clock::now();
     fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
     return 0;
 }

@@ -109,6 +109,21 @@ _SIGS.update({
 })
 
 
+_SIGS["oracle_barrier_program"] = (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP,
+                                                   ctypes.c_int, ctypes.POINTER(OracleStats)])
+
+
+def barrier_program(dep, non_dep, barriers, threads=1):
+    """Runs the barrier specification on the host executors; returns (counter, hits, violations, stats)."""
+    counter = np.zeros(1, dtype=np.uint32)
+    violations = np.zeros(1, dtype=np.uint32)
+    hits = np.zeros(max(1, dep), dtype=np.uint32)
+    st = OracleStats()
+    lib().oracle_barrier_program(dep, non_dep, barriers, counter.ctypes.data, hits.ctypes.data,
+                                 violations.ctypes.data, threads, ctypes.byref(st))
+    return int(counter[0]), hits[:dep], int(violations[0]), st
+
+
 def photon_params(**overrides):
     d = dict(PHOTON_DEFAULTS)
     d.update(overrides)
