@@ -37,8 +37,13 @@ class RunConfig(ctypes.Structure):
         ("queues", ctypes.c_uint32),
         ("coalesce", ctypes.c_uint32),
         ("max_launches", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("block_groups", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+        ("spec_param", ctypes.c_uint32),
     ]
+
+
+RUN_GRAPH = 1
 
 
 class RunStats(ctypes.Structure):
@@ -96,7 +101,8 @@ _SIGNATURES = {
     "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
 }
 
-PROGRAM_SUFFIXES = ("collatz_event", "collatz_async")
+PROGRAM_SUFFIXES = ("collatz_event", "collatz_async", "photon_event", "photon_async", "bfs_event", "bfs_async",
+                    "barrier_async")
 for _s in PROGRAM_SUFFIXES:
     _SIGNATURES.update(
         {
@@ -282,23 +288,6 @@ _SIGNATURES.update(
         ),
     }
 )
-for _s in ("barrier_async",):
-    _SIGNATURES.update(
-        {
-            f"alloc_state_{_s}": (_VP, []),
-            f"free_state_{_s}": (None, [_VP]),
-            f"load_state_device_{_s}": (None, [_VP, _VP]),
-            f"store_state_device_{_s}": (None, [_VP, _VP]),
-            f"alloc_program_{_s}": (_VP, [_VP, ctypes.c_size_t]),
-            f"free_program_{_s}": (None, [_VP]),
-            f"init_program_{_s}": (None, [_VP, ctypes.c_size_t]),
-            f"exec_program_{_s}": (None, [_VP, ctypes.c_size_t, ctypes.c_size_t]),
-            f"complete_{_s}": (ctypes.c_int, [_VP]),
-            f"clear_flags_{_s}": (None, [_VP]),
-            f"set_device_{_s}": (None, [ctypes.c_int]),
-        }
-    )
-
 
 def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
     """Runs the remapping-barrier program; returns (counter, hits[dep], violations, stats)."""
@@ -310,22 +299,6 @@ def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
     check(lib().hz_barrier_run(dep_count, non_dep_count, barrier_count, counter.ptr, hits.ptr, violations.ptr,
                                ctypes.byref(config), ctypes.byref(st)), "hz_barrier_run")
     return int(counter.numpy()[0]), hits.numpy()[:dep_count], int(violations.numpy()[0]), st.as_dict()
-for _s in ("photon_event", "photon_async"):
-    _SIGNATURES.update(
-        {
-            f"alloc_state_{_s}": (_VP, []),
-            f"free_state_{_s}": (None, [_VP]),
-            f"load_state_device_{_s}": (None, [_VP, _VP]),
-            f"store_state_device_{_s}": (None, [_VP, _VP]),
-            f"alloc_program_{_s}": (_VP, [_VP, ctypes.c_size_t]),
-            f"free_program_{_s}": (None, [_VP]),
-            f"init_program_{_s}": (None, [_VP, ctypes.c_size_t]),
-            f"exec_program_{_s}": (None, [_VP, ctypes.c_size_t, ctypes.c_size_t]),
-            f"complete_{_s}": (ctypes.c_int, [_VP]),
-            f"clear_flags_{_s}": (None, [_VP]),
-            f"set_device_{_s}": (None, [ctypes.c_int]),
-        }
-    )
 
 
 def photon_params(**overrides) -> PhotonParams:
@@ -352,3 +325,124 @@ def photon_run(kind, params: PhotonParams, tally: DeviceArray | None = None, eve
           "hz_photon_run")
     reps = tally.numpy().reshape(params.replicas, stride)
     return reps.sum(axis=0), events.numpy(), st.as_dict()
+
+
+# ------------------------------------------------------------------ batched BFS
+
+BFS_UNREACHED = 0x7FFFFFFF
+
+
+class BfsInfo(ctypes.Structure):
+    _fields_ = [
+        ("graphs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("total_vertices", ctypes.c_uint64),
+        ("total_edges", ctypes.c_uint64),
+        ("d_row_ptr", _VP),
+        ("d_col", _VP),
+        ("d_level", _VP),
+        ("build_ms", ctypes.c_double),
+    ]
+
+
+_SIGNATURES.update(
+    {
+        "hz_bfs_batch_sizes": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, _VP]),
+        "hz_bfs_batch_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, _VP, ctypes.c_uint32,
+                                               ctypes.POINTER(_VP)]),
+        "hz_bfs_batch_from_csr": (ctypes.c_int, [ctypes.c_uint32, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
+        "hz_bfs_batch_destroy": (ctypes.c_int, [_VP]),
+        "hz_bfs_batch_info": (ctypes.c_int, [_VP, ctypes.POINTER(BfsInfo)]),
+        "hz_bfs_batch_layout": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+        "hz_bfs_batch_run": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)]),
+        "hz_bfs_batch_release": (ctypes.c_int, [_VP]),
+    }
+)
+
+
+def bfs_batch_sizes(seed: int, graphs: int, min_log2: float = 14.0, max_log2: float = 22.0) -> np.ndarray:
+    """Vertex counts of an R-MAT batch (host computation in the library)."""
+    out = np.zeros(graphs, dtype=np.uint32)
+    check(lib().hz_bfs_batch_sizes(seed, graphs, min_log2, max_log2, out.ctypes.data), "hz_bfs_batch_sizes")
+    return out
+
+
+class BfsBatch:
+    """Many CSR graphs resident on the device, searched by one program per run."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        info = BfsInfo()
+        check(lib().hz_bfs_batch_info(handle, ctypes.byref(info)), "hz_bfs_batch_info")
+        self.graphs = info.graphs
+        self.total_vertices = info.total_vertices
+        self.total_edges = info.total_edges
+        self.build_ms = info.build_ms
+        self._d_row_ptr, self._d_col, self._d_level = info.d_row_ptr, info.d_col, info.d_level
+        g = self.graphs
+        self.vertices = np.zeros(g, np.uint32)
+        self.edges = np.zeros(g, np.uint64)
+        self.vertex_base = np.zeros(g, np.uint64)
+        self.edge_base = np.zeros(g, np.uint64)
+        self.row_base = np.zeros(g, np.uint64)
+        check(lib().hz_bfs_batch_layout(handle, self.vertices.ctypes.data, self.edges.ctypes.data,
+                                        self.vertex_base.ctypes.data, self.edge_base.ctypes.data,
+                                        self.row_base.ctypes.data), "hz_bfs_batch_layout")
+
+    @classmethod
+    def rmat(cls, seed: int, vertices, edge_factor: int = 16) -> "BfsBatch":
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        h = _VP()
+        check(lib().hz_bfs_batch_create(seed, len(v), v.ctypes.data, edge_factor, ctypes.byref(h)),
+              "hz_bfs_batch_create")
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, vertices, edges, row_ptr, col, sources=None) -> "BfsBatch":
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        e = np.ascontiguousarray(edges, dtype=np.uint64)
+        r = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        c = np.ascontiguousarray(col, dtype=np.int32)
+        s = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+        h = _VP()
+        check(lib().hz_bfs_batch_from_csr(len(v), v.ctypes.data, e.ctypes.data, r.ctypes.data,
+                                          c.ctypes.data if c.size else None,
+                                          None if s is None else s.ctypes.data, ctypes.byref(h)),
+              "hz_bfs_batch_from_csr")
+        return cls(h.value)
+
+    def run(self, kind="async", **cfg) -> dict:
+        k = KINDS[kind] if isinstance(kind, str) else int(kind)
+        st = RunStats()
+        config = make_config(**cfg)
+        check(lib().hz_bfs_batch_run(self.handle, k, ctypes.byref(config), ctypes.byref(st)), "hz_bfs_batch_run")
+        return st.as_dict()
+
+    def _download(self, ptr, count, dtype):
+        out = np.empty(int(count), dtype=dtype)
+        if out.size:
+            lib().harmonize_memcpy_device_to_host(out.ctypes.data, ptr, out.nbytes)
+            check_void("download")
+        return out
+
+    def levels(self) -> np.ndarray:
+        return self._download(self._d_level, self.total_vertices, np.int32)
+
+    def csr(self):
+        """(row_ptr, col) of the whole batch, as laid out on the device."""
+        return (self._download(self._d_row_ptr, self.total_vertices + self.graphs, np.int32),
+                self._download(self._d_col, self.total_edges, np.int32))
+
+    def release(self) -> None:
+        check(lib().hz_bfs_batch_release(self.handle), "hz_bfs_batch_release")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().hz_bfs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

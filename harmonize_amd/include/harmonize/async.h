@@ -29,6 +29,9 @@
 //    still produce work; the run is quiescent when both are zero.
 #pragma once
 
+#include <cstdio>
+#include <string>
+
 #include "host.h"
 #include "iter.h"
 #include "spec.h"
@@ -49,7 +52,9 @@ struct LinkQueue {
 // Run status and statistics of an AsyncProgram instance, kept in HBM.
 struct AsyncStatus {
     unsigned int flags;
-    unsigned int pad0[31];
+    unsigned int stall_site;          // first bounded wait that gave up (AsyncStall)
+    unsigned long long stall_detail;  // what it waited on
+    unsigned int pad0[28];
     int units;
     unsigned int pad1[31];
     int makers;
@@ -82,6 +87,27 @@ enum AsyncPhase : unsigned int {
 };
 
 constexpr unsigned int LINK_NONE = 0xFFFFFFFFu;
+
+// Bounded waits of the runtime; the first one to give up is recorded.
+enum AsyncStall : unsigned int {
+    STALL_NONE = 0,
+    STALL_QUEUE_PUSH = 1,     // a ring slot stayed occupied (detail: queue << 32 | slot)
+    STALL_QUEUE_POP = 2,      // a reserved ring slot stayed empty (detail: queue << 32 | slot)
+    STALL_BARRIER_WAIT = 3,   // no link became available in a barrier (detail: partial word)
+    STALL_BARRIER_INSTALL = 4,  // a fresh barrier link could not be published (detail: partial word)
+    STALL_BARRIER_LIST = 5,   // a full barrier link could not be listed (detail: list head)
+};
+
+inline const char* stall_name(unsigned int site) {
+    switch (site) {
+        case STALL_QUEUE_PUSH: return "link queue push";
+        case STALL_QUEUE_POP: return "link queue pop";
+        case STALL_BARRIER_WAIT: return "barrier await";
+        case STALL_BARRIER_INSTALL: return "barrier link install";
+        case STALL_BARRIER_LIST: return "barrier full list";
+        default: return "unknown";
+    }
+}
 
 }  // namespace hz
 
@@ -318,7 +344,10 @@ class AsyncProgram {
                     "construct the Instance with a larger arena");
             }
             if (host_status.flags & QUEUE_STALL_FLAG) {
-                throw std::runtime_error("AsyncProgram: a queue slot wait exceeded its bound");
+                char detail[64];
+                std::snprintf(detail, sizeof(detail), "%016llx", (unsigned long long)host_status.stall_detail);
+                throw std::runtime_error(std::string("AsyncProgram: a bounded wait gave up (") +
+                                         hz::stall_name(host_status.stall_site) + ", detail 0x" + detail + ")");
             }
             return (host_status.flags & COMPLETION_FLAG) != 0;
         }
@@ -347,6 +376,19 @@ class AsyncProgram {
     // ---------------------------------------------------------------- status
     __device__ void set_flags(unsigned int bits) { hz::wave::atomic_or(&_dev_ctx.status->flags, bits); }
 
+    // Records the first bounded wait that gave up and flags the run.
+    __device__ void stall(unsigned int site, unsigned long long detail) {
+        if (atomicCAS(&_dev_ctx.status->stall_site, 0u, site) == 0u) {
+            hz::wave::atomic_store(&_dev_ctx.status->stall_detail, detail);
+        }
+        set_flags(QUEUE_STALL_FLAG);
+    }
+
+    __device__ unsigned long long queue_detail(hz::LinkQueue* q, unsigned int pos) {
+        unsigned long long qi = q == _dev_ctx.free_queue ? 0xFFFFFFFFull : (unsigned long long)(q - _dev_ctx.queues);
+        return (qi << 32) | pos;
+    }
+
     __device__ void add_units(int delta) { hz::wave::atomic_add_acqrel(&_dev_ctx.status->units, delta); }
 
     // ------------------------------------------------------------ link queues
@@ -357,7 +399,7 @@ class AsyncProgram {
         uint32_t spins = 0;
         while (hz::wave::atomic_load_acquire(slot) != hz::LINK_NONE) {
             if (++spins > SLOT_SPIN_LIMIT) {
-                set_flags(QUEUE_STALL_FLAG);
+                stall(hz::STALL_QUEUE_PUSH, queue_detail(q, t));
                 return false;
             }
             hz::wave::sleep_short();
@@ -380,7 +422,7 @@ class AsyncProgram {
         uint32_t spins = 0;
         while ((id = hz::wave::atomic_load_acquire(slot)) == hz::LINK_NONE) {
             if (++spins > SLOT_SPIN_LIMIT) {
-                set_flags(QUEUE_STALL_FLAG);
+                stall(hz::STALL_QUEUE_POP, queue_detail(q, h));
                 return hz::LINK_NONE;
             }
             hz::wave::sleep_short();
@@ -921,63 +963,98 @@ class AsyncProgram {
     // (protocol described in barrier.h). link_aux[2L] = next link in a
     // barrier's full list; link_aux[2L+1] = (target << 16) | filled.
 
+    // Every lane that awaits places its promise within one retry loop whose
+    // exit is wave-uniform: a lane waiting for a link that a lane of its own
+    // wave installs must see that installation in the same pass. (With a
+    // per-lane loop the installing lane's exit branch would run only after
+    // all of the wave's lanes had left the loop.)
     template <typename BARRIER, typename OP_TYPE>
     __device__ void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
         constexpr OpDisc D = Lookup<OP_TYPE>::type::DISC;
-        if (hz::wave::atomic_load(&b.semaphore.sem) == 0u) {
-            async_call_cast<OP_TYPE>(0, promise);
-            return;
-        }
+        bool run_now = hz::wave::atomic_load(&b.semaphore.sem) == 0u;
+        bool pending = !run_now;
+        if (pending) hz::wave::atomic_add(&b.count, 1u);
         PromiseUnionType pu;
         pu.template cast<OP_TYPE>() = promise;
         uint32_t words[WORDS];
         __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
-        hz::wave::atomic_add(&b.count, 1u);
         unsigned long long* part = &b.partial_table[D];
-        for (uint32_t attempt = 0; attempt < SLOT_SPIN_LIMIT; ++attempt) {
-            unsigned long long old = hz::wave::atomic_add_acqrel(part, 1ull);
-            unsigned int L = (unsigned int)(old >> 32);
-            unsigned int s = (unsigned int)old;
-            if (L == hz::BARRIER_RELEASED) {
-                async_call_cast<OP_TYPE>(0, promise);
-                return;
+        for (uint32_t attempt = 0; hz::wave::ballot(pending) != 0ull; ++attempt) {
+            if (attempt >= SLOT_SPIN_LIMIT) {
+                if (pending) stall(hz::STALL_BARRIER_WAIT, hz::wave::atomic_load(part));
+                break;
             }
-            if (L != hz::BARRIER_NO_LINK && s < 64u) {
-                barrier_write(b, L, s, D, words);
-                return;
-            }
-            if ((L == hz::BARRIER_NO_LINK && s == 0u) || (L != hz::BARRIER_NO_LINK && s == 64u)) {
-                // This lane installs a fresh link with its promise in slot 0.
-                unsigned int fresh = arena_alloc_any();
-                if (fresh == hz::LINK_NONE) {
-                    set_flags(ARENA_EXHAUSTED_FLAG);
-                    return;
+            // While a link is being installed ((NO_LINK, s > 0) or (L, s > 64))
+            // a waiter only reads the word: reserving then would change it
+            // and starve the installer's compare-and-swap. (L, 64) is still
+            // reserved: the lane that draws 64 installs the next link.
+            bool reserve = false;
+            if (pending) {
+                unsigned long long seen = hz::wave::atomic_load(part);
+                unsigned int L = (unsigned int)(seen >> 32);
+                unsigned int s = (unsigned int)seen;
+                if (L == hz::BARRIER_RELEASED) {
+                    run_now = true;
+                    pending = false;
+                } else {
+                    reserve = L == hz::BARRIER_NO_LINK ? s == 0u : s <= 64u;
                 }
-                hz::wave::atomic_store(&_dev_ctx.link_aux[2u * fresh + 1u], 64u << 16);
-                write_arena_slot(fresh, 0u, words);
-                // The count word must be set before a lane that reads the
-                // installed link from the partial word counts into it.
-                hz::wave::fence_release();
-                unsigned long long want = ((unsigned long long)fresh << 32) | 1ull;
-                unsigned long long cur = hz::wave::atomic_load(part);
-                while (true) {
-                    unsigned int cl = (unsigned int)(cur >> 32);
-                    if (cl == hz::BARRIER_RELEASED) {  // released meanwhile: run it now
-                        queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh);
-                        async_call_cast<OP_TYPE>(0, promise);
-                        return;
+            }
+            if (reserve) {
+                unsigned long long old = hz::wave::atomic_add_acqrel(part, 1ull);
+                unsigned int L = (unsigned int)(old >> 32);
+                unsigned int s = (unsigned int)old;
+                if (L == hz::BARRIER_RELEASED) {
+                    run_now = true;
+                    pending = false;
+                } else if (L != hz::BARRIER_NO_LINK && s < 64u) {
+                    barrier_write(b, L, s, D, words);
+                    pending = false;
+                } else if ((L == hz::BARRIER_NO_LINK && s == 0u) || (L != hz::BARRIER_NO_LINK && s == 64u)) {
+                    // This lane installs a fresh link with its promise in slot 0.
+                    pending = false;
+                    unsigned int fresh = arena_alloc_any();
+                    if (fresh == hz::LINK_NONE) {
+                        set_flags(ARENA_EXHAUSTED_FLAG);
+                    } else {
+                        hz::wave::atomic_store(&_dev_ctx.link_aux[2u * fresh + 1u], 64u << 16);
+                        write_arena_slot(fresh, 0u, words);
+                        // The count word must be set before a lane that reads
+                        // the installed link from the partial word counts into it.
+                        hz::wave::fence_release();
+                        unsigned long long want = ((unsigned long long)fresh << 32) | 1ull;
+                        unsigned long long cur = hz::wave::atomic_load(part);
+                        bool installed = false, released = false;
+                        // Only this lane installs until its link is published;
+                        // the word changes meanwhile only by reservations that
+                        // were already in flight, or by a release.
+                        for (uint32_t tries = 0; tries < SLOT_SPIN_LIMIT; ++tries) {
+                            if ((unsigned int)(cur >> 32) == hz::BARRIER_RELEASED) {
+                                released = true;
+                                break;
+                            }
+                            unsigned long long seen = atomicCAS(part, cur, want);
+                            if (seen == cur) {
+                                installed = true;
+                                break;
+                            }
+                            cur = seen;
+                        }
+                        if (installed) {
+                            barrier_finish(b, fresh, D);
+                        } else if (released) {  // released meanwhile: run it now
+                            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh);
+                            run_now = true;
+                        } else {
+                            stall(hz::STALL_BARRIER_INSTALL, cur);
+                        }
                     }
-                    unsigned long long seen = atomicCAS(part, cur, want);
-                    if (seen == cur) break;
-                    cur = seen;
                 }
-                barrier_finish(b, fresh, D);
-                return;
+                // Otherwise another lane is installing a link: retry.
             }
-            // Another lane is installing a link: back off and retry.
-            hz::wave::sleep_short();
+            if (hz::wave::ballot(pending) != 0ull) hz::wave::sleep_short();
         }
-        set_flags(QUEUE_STALL_FLAG);
+        if (run_now) async_call_cast<OP_TYPE>(0, promise);
     }
 
     template <typename BARRIER>
@@ -1068,7 +1145,7 @@ class AsyncProgram {
             if (seen == h) return;
             h = seen;
         }
-        set_flags(QUEUE_STALL_FLAG);
+        stall(hz::STALL_BARRIER_LIST, ((unsigned long long)L << 32) | h);
     }
 
    public:
@@ -1110,6 +1187,8 @@ class AsyncProgram {
             ctx.free_queue->count = 0;
             hz::AsyncStatus* st = ctx.status;
             st->flags = 0;
+            st->stall_site = 0;
+            st->stall_detail = 0;
             st->units = 0;
             st->makers = 0;
             st->checkout = 0;

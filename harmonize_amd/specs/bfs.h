@@ -1,0 +1,118 @@
+// Batched breadth-first search: many graphs (jobs) in one program. Each job
+// is an int32 CSR graph with an int32 level array; the program computes
+// every job's BFS levels from its source vertex.
+//
+// Label-correcting formulation, so that the result does not depend on the
+// order in which promises run:
+//  - BfsVisit(job, v, l) relaxes the out-edges of v at level l + 1, unless
+//    v's level has meanwhile dropped below l (a better visit exists);
+//  - relaxing edge v -> w at level n does atomicMin(level[w], n) and, when
+//    that lowered the level, queues BfsVisit(job, w, n);
+//  - an edge range longer than the split threshold is halved, the upper
+//    half queued as BfsEdges(job, begin, end, n), until the rest is short:
+//    a hub's edges spread over many lanes instead of one.
+// Levels converge to the exact BFS distances (each final level is attained
+// by a visit that relaxed along a shortest path); unreached vertices keep
+// BFS_UNREACHED. The number of visits depends on the order, the levels do not.
+#pragma once
+
+#include <cstdint>
+
+constexpr int32_t BFS_UNREACHED = 0x7FFFFFFF;
+constexpr uint32_t BFS_DEFAULT_SPLIT = 64u;
+
+// One graph of the batch (32 bytes).
+struct BfsJob {
+    const int32_t* row_ptr;  // [vertices + 1], local edge offsets
+    const int32_t* col;      // [edges], local vertex ids
+    int32_t* level;          // [vertices]
+    uint32_t vertices;
+    uint32_t source;
+};
+
+struct BfsState {
+    const BfsJob* jobs;
+    util::iter::AtomicIter<unsigned int>* job_iter;  // jobs still to start
+    unsigned int job_count;
+    unsigned int split;  // edge ranges longer than this are halved
+};
+
+struct BfsVisit;
+struct BfsEdges;
+
+template <typename PROGRAM>
+__device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& j, int begin, int end, int next) {
+    const int split = (int)prog.device.split;
+    while (end - begin > split) {
+        int mid = begin + ((end - begin) >> 1);
+        prog.template async<BfsEdges>(job, mid, end, next);
+        end = mid;
+    }
+    for (int i = begin; i < end; ++i) {
+        int w = j.col[i];
+        if (hz_load(&j.level[w]) > next) {
+            int before = atomicMin(&j.level[w], next);
+            if (before > next) prog.template async<BfsVisit>(job, w, next);
+        }
+    }
+}
+
+struct BfsVisit {
+    using Type = void (*)(unsigned int job, int vertex, int level);
+
+    template <typename PROGRAM>
+    __device__ static void eval(PROGRAM prog, unsigned int job, int vertex, int level) {
+        const BfsJob j = prog.device.jobs[job];
+        if (hz_load(&j.level[vertex]) < level) return;
+        bfs_relax(prog, job, j, j.row_ptr[vertex], j.row_ptr[vertex + 1], level + 1);
+    }
+};
+
+struct BfsEdges {
+    using Type = void (*)(unsigned int job, int begin, int end, int level);
+
+    template <typename PROGRAM>
+    __device__ static void eval(PROGRAM prog, unsigned int job, int begin, int end, int level) {
+        const BfsJob j = prog.device.jobs[job];
+        bfs_relax(prog, job, j, begin, end, level);
+    }
+};
+
+#ifndef HZ_BFS_STASH
+#define HZ_BFS_STASH 16
+#endif
+
+// GROUPS: wavefront groups per workgroup (the launch's work-group size / 64).
+template <unsigned int GROUPS>
+struct BfsSpecT {
+    typedef OpUnion<BfsVisit, BfsEdges> OpSet;
+    typedef BfsState DeviceState;
+
+    static const size_t STASH_SIZE = HZ_BFS_STASH;
+    static const size_t FRAME_SIZE = 8;
+    static const size_t POOL_SIZE = 8;
+    static const size_t GROUPS_PER_BLOCK = GROUPS;
+
+    template <typename PROGRAM>
+    __device__ static void initialize(PROGRAM) {}
+
+    template <typename PROGRAM>
+    __device__ static void finalize(PROGRAM) {}
+
+    // Starts one job per thread: level[source] = 0 and a visit of the source.
+    template <typename PROGRAM>
+    __device__ static bool make_work(PROGRAM prog) {
+        util::iter::Iter<unsigned int> it = prog.device.job_iter->leap(1u);
+        unsigned int job;
+        while (it.step(job)) {
+            const BfsJob j = prog.device.jobs[job];
+            if (j.source < j.vertices) {
+                atomicMin(&j.level[j.source], 0);
+                prog.template async<BfsVisit>(job, (int)j.source, 0);
+            }
+        }
+        return !prog.device.job_iter->done();
+    }
+};
+
+using BfsSpec = BfsSpecT<1>;

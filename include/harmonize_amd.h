@@ -102,6 +102,24 @@ typedef struct hz_photon_state {
  * [0, dep + non_dep); a dependent id awaits barrier[id % barriers] with a
  * held Step and decrements that barrier's semaphore. Layout of
  * BarrierDemoState. */
+/* Batched BFS. A job is one int32 CSR graph (row_ptr[vertices + 1] of local
+ * edge offsets, col[edges] of local vertex ids) with an int32 level array;
+ * unreached vertices keep HZ_BFS_UNREACHED. */
+#define HZ_BFS_UNREACHED 0x7FFFFFFF
+typedef struct hz_bfs_job {
+    const int32_t* row_ptr;
+    const int32_t* col;
+    int32_t* level;
+    uint32_t vertices;
+    uint32_t source;
+} hz_bfs_job;
+typedef struct hz_bfs_state {
+    const hz_bfs_job* jobs;
+    void* job_iter; /* {uint32 next, uint32 limit}: jobs still to start */
+    uint32_t job_count;
+    uint32_t split; /* edge ranges longer than this are halved (0 is invalid) */
+} hz_bfs_state;
+
 typedef struct hz_barrier_state {
     uint32_t dep_count, non_dep_count, barrier_count, pad;
     void* source_id_iter;
@@ -113,7 +131,8 @@ typedef struct hz_barrier_state {
 
 /* ------------------------------------------- per-program entry points
  * For each <suffix> in collatz_event, collatz_async, photon_event,
- * photon_async, barrier_async (the barrier needs AsyncProgram):
+ * photon_async, bfs_event, bfs_async, barrier_async (the barrier needs
+ * AsyncProgram):
  *
  *   void* alloc_state_<suffix>(void);
  *       Device allocation sized for the program's device state
@@ -154,6 +173,8 @@ HZ_DECLARE_PROGRAM_ABI(collatz_event)
 HZ_DECLARE_PROGRAM_ABI(collatz_async)
 HZ_DECLARE_PROGRAM_ABI(photon_event)
 HZ_DECLARE_PROGRAM_ABI(photon_async)
+HZ_DECLARE_PROGRAM_ABI(bfs_event)
+HZ_DECLARE_PROGRAM_ABI(bfs_async)
 HZ_DECLARE_PROGRAM_ABI(barrier_async)
 
 /* ------------------------------------------------------ drivers (layer 2) */
@@ -168,8 +189,15 @@ typedef struct hz_run_config {
     uint32_t coalesce; /* async: minimum promises in an open link before it
                           runs while make_work can still add (0 = 64) */
     uint32_t max_launches; /* bound on the host launch loop (0 = none) */
-    uint32_t reserved;
+    uint32_t block_groups; /* async: wavefront groups per workgroup where the
+                              program offers variants (BFS: 1, 2 or 4; 0 = 1) */
+    uint32_t flags;        /* HZ_RUN_GRAPH: launch the pass through a captured
+                              hipGraph (BFS batches, HZ_ASYNC) */
+    uint32_t spec_param;   /* program-specific knob (BFS: longest edge range a
+                              promise relaxes before halving it; 0 = 64) */
 } hz_run_config;
+
+#define HZ_RUN_GRAPH 1u
 
 typedef struct hz_run_stats {
     uint64_t promises;        /* promises evaluated */
@@ -221,6 +249,49 @@ int hz_detmath_eval(const double* d_in, double* d_log, double* d_sqrt, uint32_t*
  * *d_violations = held steps that ran before their barrier's release. */
 int hz_barrier_run(uint32_t dep_count, uint32_t non_dep_count, uint32_t barrier_count, uint32_t* d_counter,
                    uint32_t* d_hits, uint32_t* d_violations, const hz_run_config* config, hz_run_stats* stats);
+
+/* ------------------------------------------------------- batched BFS
+ * A batch owns the CSR graphs of many jobs, their level arrays, and the
+ * program instances that run them (kept between runs, so a run with the same
+ * configuration reuses its instance and, with HZ_RUN_GRAPH, its hipGraph).
+ * Graphs are laid out back to back: graph g's levels start at vertex_base[g],
+ * its col at edge_base[g], its row_ptr at row_base[g] = vertex_base[g] + g. */
+typedef struct hz_bfs_batch hz_bfs_batch;
+typedef struct hz_bfs_info {
+    uint32_t graphs;
+    uint32_t reserved;
+    uint64_t total_vertices;
+    uint64_t total_edges;
+    const int32_t* d_row_ptr;
+    const int32_t* d_col;
+    int32_t* d_level;
+    double build_ms; /* device time of generation + CSR build */
+} hz_bfs_info;
+
+/* Vertex counts of an R-MAT batch: graph g has
+ * round(2^(min_log2 + (max_log2 - min_log2) * u_g)) vertices, u_g uniform in
+ * [0, 1) from Philox4x32-10 keyed by seed. Host-only computation. */
+int hz_bfs_batch_sizes(uint64_t seed, uint32_t graphs, double min_log2, double max_log2, uint32_t* vertices);
+
+/* Generates `graphs` R-MAT graphs (a, b, c = 0.57, 0.19, 0.19) of
+ * vertices[g] vertices and edge_factor * vertices[g] directed edges on the
+ * device, duplicates and self-loops kept, and builds their CSR with
+ * neighbours in generation order. BFS source: vertex 0. */
+int hz_bfs_batch_create(uint64_t seed, uint32_t graphs, const uint32_t* vertices, uint32_t edge_factor,
+                        hz_bfs_batch** out);
+/* A batch from host CSR arrays laid out as above (sources may be NULL: 0). */
+int hz_bfs_batch_from_csr(uint32_t graphs, const uint32_t* vertices, const uint64_t* edges, const int32_t* row_ptr,
+                          const int32_t* col, const uint32_t* sources, hz_bfs_batch** out);
+int hz_bfs_batch_destroy(hz_bfs_batch* batch);
+int hz_bfs_batch_info(const hz_bfs_batch* batch, hz_bfs_info* info);
+/* Per-graph layout (any output may be NULL). */
+int hz_bfs_batch_layout(const hz_bfs_batch* batch, uint32_t* vertices, uint64_t* edges, uint64_t* vertex_base,
+                        uint64_t* edge_base, uint64_t* row_base);
+/* BFS of every job of the batch in one program (kind HZ_ASYNC or HZ_EVENT);
+ * levels land in info.d_level. */
+int hz_bfs_batch_run(hz_bfs_batch* batch, int kind, const hz_run_config* config, hz_run_stats* stats);
+/* Frees the program instances the batch keeps for reuse. */
+int hz_bfs_batch_release(hz_bfs_batch* batch);
 
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {

@@ -12,7 +12,9 @@
 #include "collatz.h"
 #include "photon.h"
 #include "barrier_demo.h"
+#include "bfs.h"
 #include "philox.h"
+#include "rmat.h"
 
 extern "C" {
 
@@ -208,6 +210,119 @@ int oracle_barrier_program(uint32_t dep, uint32_t non_dep, uint32_t barrier_coun
     auto t0 = std::chrono::steady_clock::now();
     oracle::RunStats s = used == 1 ? oracle::HostProgram<BarrierDemoSpec>::run(st)
                                    : oracle::OmpProgram<BarrierDemoSpec>::run(st, used);
+    auto t1 = std::chrono::steady_clock::now();
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// One R-MAT edge (the generator shared with the device; its bit mapping is
+// pinned separately by a numpy restatement in the tests).
+void oracle_rmat_edge(uint64_t seed, uint32_t g, uint32_t e, uint32_t vertices, uint32_t* src, uint32_t* dst) {
+    hz::rmat::edge(g, e, vertices, hz::rmat::scale_of(vertices), (uint32_t)seed, (uint32_t)(seed >> 32), *src, *dst);
+}
+
+// CSR of an R-MAT batch built on the host: edges generated in order and
+// placed by a counting sort on the source, which keeps each source's
+// neighbours in generation order. Layout as hz_bfs_batch_create's:
+// row_ptr of graph g at vertex_base[g] + g, col at edge_base[g].
+// Graphs are numbered from first_graph in the generator's counters, so any
+// sub-range of a batch can be rebuilt on its own.
+void oracle_bfs_csr(uint64_t seed, uint32_t first_graph, uint32_t graphs, const uint32_t* vertices,
+                    uint32_t edge_factor, int32_t* row_ptr, int32_t* col) {
+    std::vector<uint64_t> vbase(graphs), ebase(graphs);
+    uint64_t vb = 0, eb = 0;
+    for (uint32_t g = 0; g < graphs; ++g) {
+        vbase[g] = vb;
+        ebase[g] = eb;
+        vb += vertices[g];
+        eb += (uint64_t)edge_factor * vertices[g];
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t gi = 0; gi < (int64_t)graphs; ++gi) {
+        const uint32_t g = (uint32_t)gi;
+        const uint32_t V = vertices[g];
+        const uint32_t E = edge_factor * V;
+        const uint32_t scale = hz::rmat::scale_of(V);
+        std::vector<uint32_t> src(E), dst(E);
+        for (uint32_t e = 0; e < E; ++e) {
+            hz::rmat::edge(first_graph + g, e, V, scale, (uint32_t)seed, (uint32_t)(seed >> 32), src[e], dst[e]);
+        }
+        int32_t* rows = row_ptr + vbase[g] + g;
+        int32_t* cols = col + ebase[g];
+        std::vector<int64_t> count((size_t)V + 1, 0);
+        for (uint32_t e = 0; e < E; ++e) count[src[e] + 1]++;
+        for (uint32_t v = 0; v < V; ++v) count[v + 1] += count[v];
+        for (uint32_t v = 0; v <= V; ++v) rows[v] = (int32_t)count[v];
+        for (uint32_t e = 0; e < E; ++e) cols[count[src[e]]++] = (int32_t)dst[e];
+    }
+}
+
+// Textbook queue BFS of each graph of a batch (layout as above, sources may
+// be NULL: vertex 0). Unreached vertices get BFS_UNREACHED.
+void oracle_bfs_levels(uint32_t graphs, const uint32_t* vertices, const uint64_t* edges, const int32_t* row_ptr,
+                       const int32_t* col, const uint32_t* sources, int32_t* levels, int threads) {
+    std::vector<uint64_t> vbase(graphs), ebase(graphs);
+    uint64_t vb = 0, eb = 0;
+    for (uint32_t g = 0; g < graphs; ++g) {
+        vbase[g] = vb;
+        ebase[g] = eb;
+        vb += vertices[g];
+        eb += edges[g];
+    }
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int64_t gi = 0; gi < (int64_t)graphs; ++gi) {
+        const uint32_t g = (uint32_t)gi;
+        const uint32_t V = vertices[g];
+        const int32_t* rows = row_ptr + vbase[g] + g;
+        const int32_t* cols = col + ebase[g];
+        int32_t* lev = levels + vbase[g];
+        for (uint32_t v = 0; v < V; ++v) lev[v] = BFS_UNREACHED;
+        uint32_t s = sources ? sources[g] : 0u;
+        if (s >= V) continue;
+        std::vector<uint32_t> queue;
+        queue.reserve(V);
+        lev[s] = 0;
+        queue.push_back(s);
+        for (size_t head = 0; head < queue.size(); ++head) {
+            uint32_t v = queue[head];
+            for (int32_t i = rows[v]; i < rows[v + 1]; ++i) {
+                uint32_t w = (uint32_t)cols[i];
+                if (lev[w] == BFS_UNREACHED) {
+                    lev[w] = lev[v] + 1;
+                    queue.push_back(w);
+                }
+            }
+        }
+    }
+}
+
+// The BFS specification itself on the host executors (threads == 1:
+// sequential; otherwise OpenMP, 0 = all cores). Same layout and outputs.
+int oracle_bfs_program(uint32_t graphs, const uint32_t* vertices, const uint64_t* edges, const int32_t* row_ptr,
+                       const int32_t* col, const uint32_t* sources, int32_t* levels, uint32_t split, int threads,
+                       oracle_run_stats* stats) {
+    std::vector<BfsJob> jobs(graphs);
+    uint64_t vb = 0, eb = 0;
+    for (uint32_t g = 0; g < graphs; ++g) {
+        jobs[g].row_ptr = row_ptr + vb + g;
+        jobs[g].col = col + eb;
+        jobs[g].level = levels + vb;
+        jobs[g].vertices = vertices[g];
+        jobs[g].source = sources ? sources[g] : 0u;
+        for (uint32_t v = 0; v < vertices[g]; ++v) levels[vb + v] = BFS_UNREACHED;
+        vb += vertices[g];
+        eb += edges[g];
+    }
+    util::iter::AtomicIter<unsigned int> iter(0u, graphs);
+    BfsState st{};
+    st.jobs = jobs.data();
+    st.job_iter = &iter;
+    st.job_count = graphs;
+    st.split = split ? split : BFS_DEFAULT_SPLIT;
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s = used == 1 ? oracle::HostProgram<BfsSpec>::run(st) : oracle::OmpProgram<BfsSpec>::run(st, used);
     auto t1 = std::chrono::steady_clock::now();
     fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
     return 0;

@@ -139,3 +139,61 @@ def photon_program(params, first=0, count=None, threads=1):
     lib().oracle_photon_program(ctypes.byref(params), first, count, tally.ctypes.data, events.ctypes.data, threads,
                                 ctypes.byref(st))
     return tally, events, st
+
+
+_SIGS.update({
+    "oracle_rmat_edge": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP]),
+    "oracle_bfs_csr": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP, ctypes.c_uint32, _VP, _VP]),
+    "oracle_bfs_levels": (None, [ctypes.c_uint32, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int]),
+    "oracle_bfs_program": (ctypes.c_int, [ctypes.c_uint32, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_uint32, ctypes.c_int,
+                                          ctypes.POINTER(OracleStats)]),
+})
+
+BFS_UNREACHED = 0x7FFFFFFF
+
+
+def rmat_edge(seed, g, e, vertices):
+    s = ctypes.c_uint32()
+    d = ctypes.c_uint32()
+    lib().oracle_rmat_edge(seed, g, e, vertices, ctypes.byref(s), ctypes.byref(d))
+    return s.value, d.value
+
+
+def bfs_csr(seed, vertices, edge_factor=16, first_graph=0):
+    """Host-built CSR of R-MAT graphs first_graph, first_graph + 1, ... of a
+    batch: (row_ptr, col, edges per graph)."""
+    v = np.ascontiguousarray(vertices, dtype=np.uint32)
+    edges = v.astype(np.uint64) * edge_factor
+    row_ptr = np.zeros(int(v.sum()) + len(v), dtype=np.int32)
+    col = np.zeros(int(edges.sum()), dtype=np.int32)
+    lib().oracle_bfs_csr(seed, first_graph, len(v), v.ctypes.data, edge_factor, row_ptr.ctypes.data,
+                         col.ctypes.data)
+    return row_ptr, col, edges
+
+
+def _bfs_args(vertices, edges, row_ptr, col, sources):
+    v = np.ascontiguousarray(vertices, dtype=np.uint32)
+    e = np.ascontiguousarray(edges, dtype=np.uint64)
+    r = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    s = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+    return v, e, r, c, s
+
+
+def bfs_levels(vertices, edges, row_ptr, col, sources=None, threads=0):
+    """Textbook queue BFS of every graph of a batch."""
+    v, e, r, c, s = _bfs_args(vertices, edges, row_ptr, col, sources)
+    out = np.zeros(int(v.sum()), dtype=np.int32)
+    lib().oracle_bfs_levels(len(v), v.ctypes.data, e.ctypes.data, r.ctypes.data, c.ctypes.data if c.size else None,
+                            _ptr(s), out.ctypes.data, threads)
+    return out
+
+
+def bfs_program(vertices, edges, row_ptr, col, sources=None, split=0, threads=1):
+    """The BFS specification on the host executors: (levels, stats)."""
+    v, e, r, c, s = _bfs_args(vertices, edges, row_ptr, col, sources)
+    out = np.zeros(int(v.sum()), dtype=np.int32)
+    st = OracleStats()
+    lib().oracle_bfs_program(len(v), v.ctypes.data, e.ctypes.data, r.ctypes.data, c.ctypes.data if c.size else None,
+                             _ptr(s), out.ctypes.data, split, threads, ctypes.byref(st))
+    return out, st
