@@ -112,6 +112,61 @@ def photon_comparison(hz, repeats: int = 3) -> dict:
     }
 
 
+BFS_SEED = 0x5EED
+BFS_GRAPHS = 64
+
+
+def bfs_comparison(hz, repeats: int = 3) -> dict:
+    """64 R-MAT graphs (2^14..2^22 vertices, 16 edges per vertex) searched by
+    one program: AsyncProgram through its captured hipGraph vs EventProgram,
+    each the best of `repeats` runs. Levels must agree between the two."""
+    sizes = hz.bfs_batch_sizes(BFS_SEED, BFS_GRAPHS, 14.0, 22.0)
+    batch = hz.BfsBatch.rmat(BFS_SEED, sizes)
+    try:
+        def best(kind, **cfg):
+            runs = []
+            for _ in range(repeats):
+                runs.append(batch.run(kind, max_launches=1 << 16, **cfg))
+            return min(runs, key=lambda r: r["elapsed_ms"]), batch.levels()
+
+        a, lev_a = best("async", flags=hz.RUN_GRAPH)
+        e, lev_e = best("event")
+        if not np.array_equal(lev_a, lev_e):
+            raise RuntimeError("bfs: async and event levels differ")
+        reached = lev_a != hz.BFS_UNREACHED
+        traversed = traversed_edges(hz, batch, reached)
+        rate = lambda r: r["promises"] / (r["elapsed_ms"] * 1e-3)
+        return {
+            "workload": f"{BFS_GRAPHS} R-MAT graphs (a,b,c = 0.57,0.19,0.19), vertices log-uniform in [2^14, 2^22] "
+                        f"(seed 0x5EED), 16 edges per vertex, int32 CSR, BFS from vertex 0 of each",
+            "vertices": int(batch.total_vertices),
+            "edges": int(batch.total_edges),
+            "reached": int(reached.sum()),
+            "traversed_edges": traversed,
+            "build_ms": batch.build_ms,
+            "async": {"visits": a["promises"], "ms": a["elapsed_ms"], "launches": a["launches"],
+                      "groups": a["groups"], "promises_per_s": rate(a), "teps": traversed / (a["elapsed_ms"] * 1e-3),
+                      "launch": "hipGraph"},
+            "event": {"visits": e["promises"], "ms": e["elapsed_ms"], "launches": e["launches"],
+                      "groups": e["groups"], "promises_per_s": rate(e), "teps": traversed / (e["elapsed_ms"] * 1e-3)},
+            "async_vs_event_time": e["elapsed_ms"] / a["elapsed_ms"],
+        }
+    finally:
+        batch.close()
+
+
+def traversed_edges(hz, batch, reached) -> int:
+    """Sum of out-degrees of reached vertices (edges a BFS must scan)."""
+    row_ptr, _ = batch.csr_rows()
+    total = 0
+    for g in range(batch.graphs):
+        v0, n = int(batch.vertex_base[g]), int(batch.vertices[g])
+        r0 = int(batch.row_base[g])
+        deg = np.diff(row_ptr[r0:r0 + n + 1].astype(np.int64))
+        total += int(deg[reached[v0:v0 + n]].sum())
+    return total
+
+
 def _leg_collatz(kind):
     def run(hz):
         starts = hz.collatz_make_starts(SEED, BITS, COUNT)
@@ -138,7 +193,20 @@ def _leg_copy(hz):
     return {"bytes": nbytes}
 
 
+def _leg_bfs(kind):
+    def run(hz):
+        batch = hz.BfsBatch.rmat(BFS_SEED, hz.bfs_batch_sizes(BFS_SEED, BFS_GRAPHS, 14.0, 22.0))
+        try:
+            # Plain stream launches, so that counter collection sees ordinary dispatches.
+            return batch.run(kind, max_launches=1 << 16)
+        finally:
+            batch.close()
+    return run
+
+
 LEGS = {
+    "bfs_async": _leg_bfs("async"),
+    "bfs_event": _leg_bfs("event"),
     "collatz_async": _leg_collatz("async"),
     "collatz_event": _leg_collatz("event"),
     "photon_async": _leg_photon("async"),
@@ -156,6 +224,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-event", action="store_true", help="skip the EventProgram comparator")
     ap.add_argument("--no-photon", action="store_true", help="skip the photon-walk Async vs Event comparison")
+    ap.add_argument("--no-bfs", action="store_true", help="skip the batched BFS Async vs Event comparison")
     ap.add_argument("--leg", choices=sorted(LEGS), help="run one leg once (for counter collection) and exit")
     args = ap.parse_args()
 
@@ -249,6 +318,9 @@ def main() -> int:
 
     if not args.no_photon:
         out["photon"] = photon_comparison(hz)
+
+    if not args.no_bfs:
+        out["bfs"] = bfs_comparison(hz)
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
     out["plain_kernel"] = {

@@ -40,6 +40,7 @@ class RunConfig(ctypes.Structure):
         ("block_groups", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
         ("spec_param", ctypes.c_uint32),
+        ("stash", ctypes.c_uint32),
     ]
 
 
@@ -57,6 +58,7 @@ class RunStats(ctypes.Structure):
         ("reserved", ctypes.c_uint32),
         ("elapsed_ms", ctypes.c_double),
         ("phase_cycles", ctypes.c_uint64 * 8),
+        ("links_run", ctypes.c_uint64),
     ]
 
     PHASES = ("cycle", "spill", "pull", "make_work", "pick", "read", "eval", "idle")
@@ -433,6 +435,10 @@ class BfsBatch:
         return (self._download(self._d_row_ptr, self.total_vertices + self.graphs, np.int32),
                 self._download(self._d_col, self.total_edges, np.int32))
 
+    def csr_rows(self):
+        """(row_ptr, None): the row offsets alone, without the column array."""
+        return self._download(self._d_row_ptr, self.total_vertices + self.graphs, np.int32), None
+
     def release(self) -> None:
         check(lib().hz_bfs_batch_release(self.handle), "hz_bfs_batch_release")
 
@@ -446,3 +452,67 @@ class BfsBatch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ autotuning
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("best", RunConfig),
+        ("best_ms", ctypes.c_double),
+        ("start_ms", ctypes.c_double),
+        ("trials", ctypes.c_uint32),
+        ("failures", ctypes.c_uint32),
+    ]
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("config", RunConfig), ("ms", ctypes.c_double)]
+
+
+_TUNE_TAIL = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(TuneResult), ctypes.POINTER(TuneTrial), ctypes.c_uint32]
+_SIGNATURES.update(
+    {
+        "hz_collatz_tune": (ctypes.c_int, [_VP, ctypes.c_uint32, _VP] + _TUNE_TAIL),
+        "hz_photon_tune": (ctypes.c_int, [ctypes.POINTER(PhotonParams), _VP, _VP] + _TUNE_TAIL),
+        "hz_bfs_batch_tune": (ctypes.c_int, [_VP] + _TUNE_TAIL),
+    }
+)
+
+
+def _config_dict(c: RunConfig) -> dict:
+    return {name: int(getattr(c, name)) for name, _ in RunConfig._fields_}
+
+
+def _tune_out(res: TuneResult, log, n: int) -> dict:
+    return {
+        "best": _config_dict(res.best),
+        "best_ms": res.best_ms,
+        "start_ms": res.start_ms,
+        "trials": res.trials,
+        "failures": res.failures,
+        "log": [(_config_dict(log[i].config), log[i].ms) for i in range(min(n, res.trials))],
+    }
+
+
+def tune_collatz(steps: DeviceArray, starts: DeviceArray | None, count: int, rounds: int = 2, reps: int = 1) -> dict:
+    res, log = TuneResult(), (TuneTrial * 256)()
+    check(lib().hz_collatz_tune(starts.ptr if starts is not None else None, count, steps.ptr, rounds, reps,
+                                ctypes.byref(res), log, 256), "hz_collatz_tune")
+    return _tune_out(res, log, 256)
+
+
+def tune_photon(params: PhotonParams, rounds: int = 2, reps: int = 1) -> dict:
+    tally = DeviceArray(params.replicas * (params.bins + PHOTON_EXTRA_SLOTS), np.float64)
+    events = DeviceArray(params.photons, np.uint32)
+    res, log = TuneResult(), (TuneTrial * 256)()
+    check(lib().hz_photon_tune(ctypes.byref(params), tally.ptr, events.ptr, rounds, reps, ctypes.byref(res), log, 256),
+          "hz_photon_tune")
+    return _tune_out(res, log, 256)
+
+
+def tune_bfs(batch: "BfsBatch", rounds: int = 2, reps: int = 1) -> dict:
+    res, log = TuneResult(), (TuneTrial * 256)()
+    check(lib().hz_bfs_batch_tune(batch.handle, rounds, reps, ctypes.byref(res), log, 256), "hz_bfs_batch_tune")
+    return _tune_out(res, log, 256)

@@ -14,8 +14,6 @@
 #include "bfs.h"
 
 using BfsAsync1 = AsyncProgram<BfsSpecT<1>>;
-using BfsAsync2 = AsyncProgram<BfsSpecT<2>>;
-using BfsAsync4 = AsyncProgram<BfsSpecT<4>>;
 using BfsEvent = EventProgram<BfsSpecT<1>>;
 
 static_assert(sizeof(BfsJob) == 32, "BfsJob layout");
@@ -282,6 +280,7 @@ struct AsyncRunner : Runner {
             stats->groups = (uint32_t)(blocks_for<P>(groups) * (P::WORK_GROUP_SIZE / 64));
             stats->elapsed_ms = ms;
             for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
+            stats->links_run = s.links_run;
         }
     }
 };
@@ -322,20 +321,34 @@ void run_event(hz_bfs_batch& b, const hz_run_config& c, hz_run_stats* stats) {
 }
 
 std::vector<uint64_t> runner_key(const hz_run_config& c) {
-    return {c.groups, c.cycles, c.arena, c.queues, c.coalesce, c.max_launches, c.block_groups, c.flags, c.spec_param};
+    return {c.groups, c.cycles, c.arena, c.queues, c.coalesce, c.max_launches, c.block_groups, c.flags, c.spec_param,
+            c.stash};
 }
 
-std::unique_ptr<Runner> make_async_runner(hz_bfs_batch& b, const hz_run_config& c) {
+template <unsigned int STASH>
+std::unique_ptr<Runner> make_async_runner_stash(hz_bfs_batch& b, const hz_run_config& c) {
     switch (c.block_groups) {
         case 0:
         case 1:
-            return std::make_unique<AsyncRunner<BfsAsync1>>(b, c);
+            return std::make_unique<AsyncRunner<AsyncProgram<BfsSpecT<1, STASH>>>>(b, c);
         case 2:
-            return std::make_unique<AsyncRunner<BfsAsync2>>(b, c);
+            return std::make_unique<AsyncRunner<AsyncProgram<BfsSpecT<2, STASH>>>>(b, c);
         case 4:
-            return std::make_unique<AsyncRunner<BfsAsync4>>(b, c);
+            return std::make_unique<AsyncRunner<AsyncProgram<BfsSpecT<4, STASH>>>>(b, c);
         default:
             throw std::invalid_argument("hz_bfs_batch_run: block_groups must be 1, 2 or 4");
+    }
+}
+
+std::unique_ptr<Runner> make_async_runner(hz_bfs_batch& b, const hz_run_config& c) {
+    switch (c.stash) {
+        case 0:
+        case 16:
+            return make_async_runner_stash<16>(b, c);
+        case 8:
+            return make_async_runner_stash<8>(b, c);
+        default:
+            throw std::invalid_argument("hz_bfs_batch_run: stash must be 8 or 16 links");
     }
 }
 

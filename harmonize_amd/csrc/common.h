@@ -129,6 +129,7 @@ void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uin
         stats->groups = (uint32_t)(blocks_for<P>(groups) * (P::WORK_GROUP_SIZE / 64));
         stats->elapsed_ms = ms;
         for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
+        stats->links_run = s.links_run;
     }
 }
 

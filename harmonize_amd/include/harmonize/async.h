@@ -59,6 +59,8 @@ struct AsyncStatus {
     unsigned int pad1[31];
     int makers;
     unsigned int pad2[31];
+    int hungry;  // groups with no work of their own, waiting on the queues
+    unsigned int pad4[31];
     unsigned int checkout;
     unsigned int claim;
     unsigned int pad3[30];
@@ -68,6 +70,7 @@ struct AsyncStatus {
     unsigned long long int make_work_calls;
     unsigned long long int launches;
     unsigned long long int idle_polls;
+    unsigned long long int links_run;  // links evaluated (promises / links_run = mean lanes busy)
     // Shader-clock cycles per scheduler phase, summed over groups; filled only
     // when the runtime is built with HZ_PHASE_TIMING (the counterpart of the
     // reference's HRM_TIME phase timers, preamble/mod.h:59-67).
@@ -148,6 +151,7 @@ class AsyncProgram {
         (uint32_t)hz::spec::GROUPS_PER_BLOCKOr<PROGRAM_SPEC, 1>::value;
     static_assert(GROUPS_PER_BLOCK >= 1 && GROUPS_PER_BLOCK <= 16, "GROUPS_PER_BLOCK must be in [1, 16]");
     static const size_t WORK_GROUP_SIZE = 64 * GROUPS_PER_BLOCK;
+    static constexpr bool SHARE_WORK = hz::spec::SHARE_WORKOr<PROGRAM_SPEC, 0>::value != 0;
     // Groups live for the whole launch (specifications may keep group-level
     // partial results, e.g. LDS tallies, and flush them in finalize).
     static constexpr bool PERSISTENT = true;
@@ -218,8 +222,11 @@ class AsyncProgram {
         unsigned long long int fills;
         unsigned long long int made;
         unsigned long long int idle;
+        unsigned long long int runs;
         uint32_t idle_streak;
         uint32_t queue_check;  // cycles until the device queues are consulted again
+        uint32_t share_check;  // cycles until the hungry count is consulted again
+        uint32_t hungry;       // this group is counted in status->hungry
 #ifdef HZ_PHASE_TIMING
         unsigned long long int phase[8];
 #endif
@@ -723,6 +730,7 @@ class AsyncProgram {
                 g.free_mask |= (1u << L);
                 g.ready_mask &= ~(1u << L);
                 g.executed += cnt;
+                g.runs += 1;
             }
             hz::wave::sync();
         }
@@ -744,6 +752,64 @@ class AsyncProgram {
     }
 
     enum CycleResult { RUN = 0, DONE = 1 };
+
+    // Work sharing (specifications with SHARE_WORK). A group's promises stay
+    // in its LDS stash until it runs short of free links, so work that grows
+    // from a few seeds (a BFS frontier) would stay with the few groups that
+    // started it. Groups without work count themselves in status->hungry; a
+    // group about to run link `keep` that holds other links consults that
+    // count every SHARE_CHECK_PERIOD cycles and spills up to that many of
+    // them (full links first, then its fullest open link) to the queues.
+    static constexpr uint32_t SHARE_CHECK_PERIOD = 4;
+
+    __device__ void set_hungry(bool h) {
+        if constexpr (!SHARE_WORK) return;
+        GroupContext& g = _grp_ctx;
+        if ((hz::wave::first(g.hungry) != 0) == h) return;
+        if (hz::wave::lane() == 0) {
+            g.hungry = h ? 1u : 0u;
+            hz::wave::atomic_add(&_dev_ctx.status->hungry, h ? 1 : -1);
+        }
+        hz::wave::sync();
+    }
+
+    __device__ void share_links(uint32_t keep) {
+        if constexpr (!SHARE_WORK) return;
+        GroupContext& g = _grp_ctx;
+        const uint32_t others = ALL_LINKS & ~hz::wave::first(g.free_mask) & ~(1u << keep);
+        if (others == 0) return;
+        uint32_t countdown = hz::wave::first(g.share_check);
+        if (hz::wave::lane() == 0) g.share_check = countdown == 0 ? SHARE_CHECK_PERIOD - 1 : countdown - 1;
+        hz::wave::sync();
+        if (countdown != 0) return;
+        int hungry = 0;
+        if (hz::wave::lane() == 0) hungry = hz::wave::atomic_load(&_dev_ctx.status->hungry);
+        hungry = hz::wave::first(hungry);
+        for (int n = 0; n < hungry; ++n) {
+            uint32_t L = hz::LINK_NONE;
+            uint32_t rm = hz::wave::first(g.ready_mask) & ~(1u << keep);
+            if (rm != 0) {
+                L = lowest(rm);
+            } else {
+                uint32_t best = 0;
+                for (uint32_t d = 0; d < OPS; ++d) {
+                    uint32_t o = hz::wave::first(g.open[d]);
+                    if (o == hz::LINK_NONE || o == keep) continue;
+                    uint32_t c = hz::wave::first(g.link_count[o]);
+                    if (c > best) {
+                        best = c;
+                        L = o;
+                    }
+                }
+                if (L != hz::LINK_NONE) {
+                    uint32_t op = hz::wave::first(g.link_op[L]);
+                    if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
+                    hz::wave::sync();
+                }
+            }
+            if (L == hz::LINK_NONE || !spill_link(L)) break;
+        }
+    }
 
     // One scheduling cycle (reference exec_cycle, program/async.h:2155-2319).
     // Without a ready link, queued links take priority over making new work
@@ -807,6 +873,8 @@ class AsyncProgram {
 
         if (L != hz::LINK_NONE) {
             if (hz::wave::lane() == 0) g.idle_streak = 0;
+            set_hungry(false);
+            share_links(L);
             run_link(L);
             return RUN;
         }
@@ -815,6 +883,7 @@ class AsyncProgram {
         HZ_PHASE(PHASE_IDLE);
         if (pull_links(1) > 0) return RUN;
         if (hz::wave::first(g.can_make_work)) return RUN;
+        set_hungry(true);
         uint32_t streak = hz::wave::first(g.idle_streak);
         if (streak >= IDLE_EXIT_POLLS) return DONE;
         int makers = 0, units = 0;
@@ -830,7 +899,13 @@ class AsyncProgram {
         units = hz::wave::first(units);
         flags = hz::wave::first(flags);
         if ((makers == 0 && units == 0) || (flags & ~COMPLETION_FLAG) != 0) return DONE;
-        hz::wave::sleep_short();
+        // Back off while idle: each poll reads the queue and status lines
+        // that busy groups update, so thousands of idle wavefronts polling at
+        // full rate slow everyone's atomics. The pause grows with the idle
+        // streak up to about 8K shader cycles.
+        uint32_t pauses = streak / 16u + 1u;
+        if (pauses > 32u) pauses = 32u;
+        for (uint32_t i = 0; i < pauses; ++i) __builtin_amdgcn_s_sleep(4);
         return RUN;
     }
 
@@ -845,9 +920,11 @@ class AsyncProgram {
             g.can_make_work = 1;
             g.maker = 1;
             g.home = blockIdx.x % _dev_ctx.queue_count;
-            g.executed = g.spills = g.fills = g.made = g.idle = 0;
+            g.executed = g.spills = g.fills = g.made = g.idle = g.runs = 0;
             g.idle_streak = 0;
             g.queue_check = 0;
+            g.share_check = 0;
+            g.hungry = 0;
 #ifdef HZ_PHASE_TIMING
             for (int i = 0; i < 8; ++i) g.phase[i] = 0;
 #endif
@@ -881,6 +958,10 @@ class AsyncProgram {
             }
         }
         if (lane == 0) {
+            if (g.hungry) {
+                g.hungry = 0;
+                hz::wave::atomic_add(&_dev_ctx.status->hungry, -1);
+            }
             if (g.holding) {
                 g.holding = 0;
                 add_units(-1);
@@ -896,6 +977,7 @@ class AsyncProgram {
             }
             hz::AsyncStatus* st = _dev_ctx.status;
             hz::wave::atomic_add(&st->promises, g.executed);
+            hz::wave::atomic_add(&st->links_run, g.runs);
             if (g.spills) hz::wave::atomic_add(&st->spills, g.spills);
             if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
             if (g.made) hz::wave::atomic_add(&st->make_work_calls, g.made);
@@ -1191,9 +1273,11 @@ class AsyncProgram {
             st->stall_detail = 0;
             st->units = 0;
             st->makers = 0;
+            st->hungry = 0;
             st->checkout = 0;
             st->claim = 0;
             st->promises = st->spills = st->fills = st->make_work_calls = st->launches = st->idle_polls = 0;
+            st->links_run = 0;
             for (int i = 0; i < 8; ++i) st->phase_cycles[i] = 0;
         }
     }

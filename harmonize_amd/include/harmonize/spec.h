@@ -49,6 +49,9 @@ HZ_CONST_OR(GROUP_SIZE)
 // Groups (wavefronts) launched per workgroup by the async program type; they
 // share the workgroup's __shared__ memory. Not a reference constant.
 HZ_CONST_OR(GROUPS_PER_BLOCK)
+// Nonzero: a group that holds several links hands some to groups that have
+// none (for work that grows from few seeds). Not a reference constant.
+HZ_CONST_OR(SHARE_WORK)
 #undef HZ_CONST_OR
 
 }  // namespace spec

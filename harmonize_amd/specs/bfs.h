@@ -8,9 +8,10 @@
 //    v's level has meanwhile dropped below l (a better visit exists);
 //  - relaxing edge v -> w at level n does atomicMin(level[w], n) and, when
 //    that lowered the level, queues BfsVisit(job, w, n);
-//  - an edge range longer than the split threshold is halved, the upper
-//    half queued as BfsEdges(job, begin, end, n), until the rest is short:
-//    a hub's edges spread over many lanes instead of one.
+//  - an edge range longer than the split threshold is cut into at most
+//    BFS_FANOUT pieces (none shorter than the threshold), all but the first
+//    queued as BfsEdges(job, begin, end, n), until the rest is short: a
+//    hub's edges spread over many lanes within a few rounds of splitting.
 // Levels converge to the exact BFS distances (each final level is attained
 // by a visit that relaxed along a shortest path); unreached vertices keep
 // BFS_UNREACHED. The number of visits depends on the order, the levels do not.
@@ -20,6 +21,7 @@
 
 constexpr int32_t BFS_UNREACHED = 0x7FFFFFFF;
 constexpr uint32_t BFS_DEFAULT_SPLIT = 64u;
+constexpr int BFS_FANOUT = 16;
 
 // One graph of the batch (32 bytes).
 struct BfsJob {
@@ -34,7 +36,7 @@ struct BfsState {
     const BfsJob* jobs;
     util::iter::AtomicIter<unsigned int>* job_iter;  // jobs still to start
     unsigned int job_count;
-    unsigned int split;  // edge ranges longer than this are halved
+    unsigned int split;  // edge ranges longer than this are split
 };
 
 struct BfsVisit;
@@ -44,9 +46,12 @@ template <typename PROGRAM>
 __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& j, int begin, int end, int next) {
     const int split = (int)prog.device.split;
     while (end - begin > split) {
-        int mid = begin + ((end - begin) >> 1);
-        prog.template async<BfsEdges>(job, mid, end, next);
-        end = mid;
+        int piece = (end - begin + BFS_FANOUT - 1) / BFS_FANOUT;
+        if (piece < split) piece = split;
+        for (long long b = (long long)begin + piece; b < end; b += piece) {
+            prog.template async<BfsEdges>(job, (int)b, end - b > piece ? (int)b + piece : end, next);
+        }
+        end = begin + piece;
     }
     for (int i = begin; i < end; ++i) {
         int w = j.col[i];
@@ -78,20 +83,19 @@ struct BfsEdges {
     }
 };
 
-#ifndef HZ_BFS_STASH
-#define HZ_BFS_STASH 16
-#endif
-
-// GROUPS: wavefront groups per workgroup (the launch's work-group size / 64).
-template <unsigned int GROUPS>
+// GROUPS: wavefront groups per workgroup (the launch's work-group size / 64);
+// STASH: LDS links per group (fewer links, more resident wavefronts).
+template <unsigned int GROUPS, unsigned int STASH = 16>
 struct BfsSpecT {
     typedef OpUnion<BfsVisit, BfsEdges> OpSet;
     typedef BfsState DeviceState;
 
-    static const size_t STASH_SIZE = HZ_BFS_STASH;
+    static const size_t STASH_SIZE = STASH;
     static const size_t FRAME_SIZE = 8;
     static const size_t POOL_SIZE = 8;
     static const size_t GROUPS_PER_BLOCK = GROUPS;
+    // A search grows from one vertex per graph: hand links to idle groups.
+    static const size_t SHARE_WORK = 1;
 
     template <typename PROGRAM>
     __device__ static void initialize(PROGRAM) {}

@@ -194,7 +194,9 @@ typedef struct hz_run_config {
     uint32_t flags;        /* HZ_RUN_GRAPH: launch the pass through a captured
                               hipGraph (BFS batches, HZ_ASYNC) */
     uint32_t spec_param;   /* program-specific knob (BFS: longest edge range a
-                              promise relaxes before halving it; 0 = 64) */
+                              promise relaxes before splitting it; 0 = 64) */
+    uint32_t stash;        /* async: LDS links per group where the program
+                              offers variants (BFS: 8 or 16; 0 = 16) */
 } hz_run_config;
 
 #define HZ_RUN_GRAPH 1u
@@ -212,6 +214,9 @@ typedef struct hz_run_stats {
                                  shader cycles summed over groups per phase
                                  (cycle, spill, pull, make_work, pick, read,
                                  eval, idle); zero otherwise */
+    uint64_t links_run;       /* async: links evaluated; promises / links_run
+                                 is the mean number of busy lanes per
+                                 evaluation (64 = perfect coalescing) */
 } hz_run_stats;
 
 /* Runs the Collatz program over `count` starts (d_starts may be NULL: start
@@ -292,6 +297,34 @@ int hz_bfs_batch_layout(const hz_bfs_batch* batch, uint32_t* vertices, uint64_t*
 int hz_bfs_batch_run(hz_bfs_batch* batch, int kind, const hz_run_config* config, hz_run_stats* stats);
 /* Frees the program instances the batch keeps for reuse. */
 int hz_bfs_batch_release(hz_bfs_batch* batch);
+
+/* ------------------------------------------------------------ autotuning
+ * Coordinate descent over hz_run_config: each round tries, axis by axis,
+ * every listed value with the other fields fixed, keeping the fastest
+ * (minimum over `reps` runs); configurations that fail to run the workload
+ * are skipped. Axes: groups, queues, coalesce, arena (smallest within 3% of
+ * the fastest) for every program; BFS adds block_groups, stash and the edge
+ * split threshold and tunes the hipGraph-launched pass. The workload's
+ * outputs hold the last trial's results. `log` (may be NULL) receives up to
+ * log_capacity trials in order. */
+typedef struct hz_tune_result {
+    hz_run_config best;
+    double best_ms;   /* time of the best configuration */
+    double start_ms;  /* time of the starting configuration (-1: it failed) */
+    uint32_t trials;  /* runs made */
+    uint32_t failures;
+} hz_tune_result;
+typedef struct hz_tune_trial {
+    hz_run_config config;
+    double ms; /* -1: the run failed */
+} hz_tune_trial;
+
+int hz_collatz_tune(const uint64_t* d_starts, uint32_t count, uint32_t* d_steps, uint32_t rounds, uint32_t reps,
+                    hz_tune_result* result, hz_tune_trial* log, uint32_t log_capacity);
+int hz_photon_tune(const hz_photon_params* params, double* d_tally, uint32_t* d_events, uint32_t rounds,
+                   uint32_t reps, hz_tune_result* result, hz_tune_trial* log, uint32_t log_capacity);
+int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tune_result* result,
+                      hz_tune_trial* log, uint32_t log_capacity);
 
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {

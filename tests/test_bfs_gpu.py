@@ -46,6 +46,8 @@ def test_csr_matches_host_build(hz, small_batch):
         ("async", {}),
         ("async", {"block_groups": 2}),
         ("async", {"block_groups": 4}),
+        ("async", {"stash": 8}),
+        ("async", {"stash": 8, "block_groups": 4, "flags": 1}),
         ("async", {"queues": 1, "coalesce": 16}),
         ("async", {"spec_param": 1}),
         ("async", {"spec_param": 1024, "queues": 16}),
@@ -119,3 +121,15 @@ def test_rmat_batch_64_graphs(hz):
         c_dev = col[b.edge_base[g]: b.edge_base[g] + b.edges[g]]
         assert np.array_equal(r_dev, r_host) and np.array_equal(c_dev, c_host), f"graph {g}"
     b.close()
+
+
+def test_tuner_picks_a_working_configuration(hz, small_batch):
+    b, want = small_batch
+    res = hz.tune_bfs(b, rounds=1, reps=1)
+    assert res["trials"] == len(res["log"]) and res["trials"] > 10
+    assert res["best_ms"] > 0 and (res["start_ms"] < 0 or res["best_ms"] <= res["start_ms"])
+    best = {k: v for k, v in res["best"].items() if v}
+    st = b.run("async", **best)
+    assert np.array_equal(b.levels(), want)
+    assert st["launches"] >= 1
+    b.release()

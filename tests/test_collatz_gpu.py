@@ -133,3 +133,18 @@ def test_reference_shaped_entry_points(suffix, size, cycles):
     getattr(L, f"free_state_{suffix}")(dev_state)
     ref, _ = O.collatz_steps(None, count=n)
     assert np.array_equal(steps.numpy(), ref)
+
+
+@pytest.mark.gpu
+def test_tuner_collatz_small():
+    import harmonize_amd as hz
+
+    n = 1 << 16
+    starts = hz.collatz_make_starts(0x5EED, 40, n)
+    steps = hz.DeviceArray(n, np.uint32)
+    res = hz.tune_collatz(steps, starts, n, rounds=1)
+    assert res["trials"] == len(res["log"]) and res["best_ms"] > 0
+    best = {k: v for k, v in res["best"].items() if v}
+    hz.collatz_run("async", steps, starts, n, **best)
+    want, _ = O.collatz_steps(starts.numpy())
+    assert np.array_equal(steps.numpy(), want)

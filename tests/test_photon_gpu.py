@@ -114,3 +114,14 @@ def test_branching_heavy_variant():
         assert np.array_equal(events, ref_e)
         assert st["promises"] == ref_st.promises
         assert_tally_close(tally, ref_t)
+
+
+def test_tuner_photon_small():
+    n = 50000
+    res = hz.tune_photon(hz.photon_params(photons=n), rounds=1)
+    assert res["trials"] == len(res["log"]) and res["best_ms"] > 0
+    best = {k: v for k, v in res["best"].items() if v}
+    tally, events, st = hz.photon_run("async", hz.photon_params(photons=n), max_launches=100000, **best)
+    ref_t, ref_e, ref_st = O.photon_program(O.photon_params(photons=n), threads=1)
+    assert np.array_equal(events, ref_e)
+    assert_tally_close(tally, ref_t)
