@@ -516,3 +516,171 @@ def tune_bfs(batch: "BfsBatch", rounds: int = 2, reps: int = 1) -> dict:
     res, log = TuneResult(), (TuneTrial * 256)()
     check(lib().hz_bfs_batch_tune(batch.handle, rounds, reps, ctypes.byref(res), log, 256), "hz_bfs_batch_tune")
     return _tune_out(res, log, 256)
+
+
+# ------------------------------------------------------------------ serving
+
+JOB_COLLATZ, JOB_PHOTON, JOB_BFS = 0, 1, 2
+JOB_TYPES = {"collatz": JOB_COLLATZ, "photon": JOB_PHOTON, "bfs": JOB_BFS}
+JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, -1
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_uint64),
+        ("arena_links", ctypes.c_uint64),
+        ("slots", ctypes.c_uint32),
+        ("max_pending", ctypes.c_uint32),
+        ("keep_outputs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("size", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("type", ctypes.c_uint32),
+        ("size", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("slot", ctypes.c_uint32),
+        ("promises", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+        ("groups", ctypes.c_uint32),
+        ("submit_ms", ctypes.c_double),
+        ("start_ms", ctypes.c_double),
+        ("end_ms", ctypes.c_double),
+        ("state_bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerInfo(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("pending", ctypes.c_uint32),
+        ("jobs", ctypes.c_uint64),
+        ("pool_bytes", ctypes.c_uint64),
+        ("pool_used", ctypes.c_uint64),
+        ("pool_largest_free", ctypes.c_uint64),
+        ("groups_collatz", ctypes.c_uint32),
+        ("groups_photon", ctypes.c_uint32),
+        ("groups_bfs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+_SIGNATURES.update(
+    {
+        "hz_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+        "hz_server_destroy": (ctypes.c_int, [_VP]),
+        "hz_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),
+        "hz_server_progress": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+        "hz_server_drain": (ctypes.c_int, [_VP, ctypes.c_double]),
+        "hz_server_job": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
+        "hz_server_output": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint32, _VP, ctypes.c_uint64]),
+        "hz_server_release": (ctypes.c_int, [_VP, ctypes.c_uint64]),
+        "hz_server_info": (ctypes.c_int, [_VP, ctypes.POINTER(ServerInfo)]),
+        "hz_server_run_trace": (ctypes.c_int, [_VP, ctypes.POINTER(Job), _VP, ctypes.c_uint32, ctypes.c_double,
+                                               ctypes.POINTER(JobResult)]),
+        "hz_pool_test": (ctypes.c_int, [ctypes.c_uint64, _VP, ctypes.c_uint32, _VP]),
+    }
+)
+
+
+def pool_test(size: int, ops) -> np.ndarray:
+    """Runs the server's job-state allocator on the host (no GPU): ops >= 0
+    allocate, -k-1 frees allocation k. Returns offsets / largest free blocks."""
+    o = np.ascontiguousarray(ops, dtype=np.int64)
+    out = np.zeros(len(o), dtype=np.uint64)
+    check(lib().hz_pool_test(size, o.ctypes.data, len(o), out.ctypes.data), "hz_pool_test")
+    return out
+
+
+def _job(kind, size: int, seed: int) -> Job:
+    t = JOB_TYPES[kind] if isinstance(kind, str) else int(kind)
+    return Job(t, int(size), int(seed))
+
+
+class Server:
+    """A request server running Collatz, photon and BFS jobs side by side."""
+
+    OUTPUTS = {
+        JOB_COLLATZ: [(np.uint32, 1), (np.uint64, 1)],
+        JOB_PHOTON: [(np.uint32, 1), (np.float64, None)],
+        JOB_BFS: [(np.int32, 1), (np.int32, 1), (np.int32, 16)],
+    }
+
+    def __init__(self, pool_bytes: int = 0, slots: int = 0, arena_links: int = 0, keep_outputs: bool = False,
+                 max_pending: int = 0):
+        cfg = ServerConfig(pool_bytes, arena_links, slots, max_pending, 1 if keep_outputs else 0, 0)
+        h = _VP()
+        check(lib().hz_server_create(ctypes.byref(cfg), ctypes.byref(h)), "hz_server_create")
+        self.handle = h.value
+
+    def submit(self, kind, size: int, seed: int) -> int:
+        i = ctypes.c_uint64()
+        j = _job(kind, size, seed)
+        check(lib().hz_server_submit(self.handle, ctypes.byref(j), ctypes.byref(i)), "hz_server_submit")
+        return i.value
+
+    def progress(self):
+        r, p = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().hz_server_progress(self.handle, ctypes.byref(r), ctypes.byref(p)), "hz_server_progress")
+        return r.value, p.value
+
+    def drain(self, timeout_ms: float = 600000.0) -> None:
+        check(lib().hz_server_drain(self.handle, timeout_ms), "hz_server_drain")
+
+    def job(self, i: int) -> dict:
+        r = JobResult()
+        check(lib().hz_server_job(self.handle, i, ctypes.byref(r)), "hz_server_job")
+        return r.as_dict()
+
+    def output(self, i: int, which: int = 0) -> np.ndarray:
+        info = self.job(i)
+        dtype, per = self.OUTPUTS[info["type"]][which]
+        if info["type"] == JOB_PHOTON and which == 1:
+            count = 64 * (1000 + PHOTON_EXTRA_SLOTS)
+        elif info["type"] == JOB_BFS and which == 1:
+            count = info["size"] + 1
+        else:
+            count = info["size"] * per
+        out = np.empty(count, dtype=dtype)
+        check(lib().hz_server_output(self.handle, i, which, out.ctypes.data, out.nbytes), "hz_server_output")
+        return out
+
+    def release(self, i: int) -> None:
+        check(lib().hz_server_release(self.handle, i), "hz_server_release")
+
+    def info(self) -> dict:
+        s = ServerInfo()
+        check(lib().hz_server_info(self.handle, ctypes.byref(s)), "hz_server_info")
+        return {name: getattr(s, name) for name, _ in s._fields_ if name != "reserved"}
+
+    def run_trace(self, jobs, arrival_ms, timeout_ms: float = 600000.0) -> list:
+        """Replays (kind, size, seed) jobs arriving at arrival_ms (sorted, ms
+        after the call); returns one result dict per job."""
+        n = len(jobs)
+        arr = (Job * n)(*[_job(*j) for j in jobs])
+        t = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        res = (JobResult * n)()
+        check(lib().hz_server_run_trace(self.handle, arr, t.ctypes.data, n, timeout_ms, res), "hz_server_run_trace")
+        return [r.as_dict() for r in res]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().hz_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

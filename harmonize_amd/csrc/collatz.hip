@@ -5,6 +5,9 @@
 #include "common.h"
 #include "collatz.h"
 #include "philox.h"
+#include "collatz_kernels.h"
+
+using hz::collatz::collatz_starts_kernel;
 
 using CollatzEvent = EventProgram<CollatzSpec>;
 using CollatzAsync = AsyncProgram<CollatzSpec>;
@@ -38,16 +41,6 @@ __global__ void __launch_bounds__(256)
             s++;
         }
         steps[i] = overflow ? COLLATZ_OVERFLOW : s;
-    }
-}
-
-__global__ void __launch_bounds__(256)
-    collatz_starts_kernel(unsigned long long seed, unsigned int bits, unsigned int count, unsigned long long* out) {
-    const unsigned long long modulus = (1ull << bits) - 1ull;
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        hz::philox::Ctr r = hz::philox::philox4x32_10(i, 0u, 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
-        unsigned long long x = ((unsigned long long)r.v[1] << 32) | r.v[0];
-        out[i] = x % modulus + 1ull;
     }
 }
 

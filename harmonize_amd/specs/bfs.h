@@ -53,7 +53,23 @@ __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& 
         }
         end = begin + piece;
     }
-    for (int i = begin; i < end; ++i) {
+    // Four edges at a time: their column reads, level reads and atomics are
+    // independent, so a lane keeps four memory chains in flight.
+    int i = begin;
+    for (; i + 4 <= end; i += 4) {
+        int w[4], seen[4], before[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = j.col[i + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) seen[k] = hz_load(&j.level[w[k]]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) before[k] = seen[k] > next ? atomicMin(&j.level[w[k]], next) : next;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (before[k] > next) prog.template async<BfsVisit>(job, w[k], next);
+        }
+    }
+    for (; i < end; ++i) {
         int w = j.col[i];
         if (hz_load(&j.level[w]) > next) {
             int before = atomicMin(&j.level[w], next);

@@ -326,6 +326,97 @@ int hz_photon_tune(const hz_photon_params* params, double* d_tally, uint32_t* d_
 int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tune_result* result,
                       hz_tune_trial* log, uint32_t log_capacity);
 
+/* ------------------------------------------------------------- serving
+ * A request server for Collatz, photon-walk and BFS jobs. Up to `slots` jobs
+ * run at once, each on its own stream with the slot's share of the device;
+ * the program instances of every slot are created with the server, so
+ * admission allocates nothing. Job state lives in a device pool; pending
+ * jobs are admitted in arrival order when a slot is free and their state
+ * fits. Jobs:
+ *   HZ_JOB_COLLATZ  size starts from Philox4x32-10 keyed by seed, uniform in
+ *                   [1, 2^32) (hz_collatz_make_starts with bits = 32);
+ *                   output 0: u32 steps, output 1: u64 starts.
+ *   HZ_JOB_PHOTON   the photon walk with hz_photon_defaults except photons =
+ *                   size, seed = seed, replicas = 64; output 0: u32 events per
+ *                   photon, output 1: the 64 tally replicas (doubles).
+ *   HZ_JOB_BFS      an R-MAT graph of size vertices and 16 x size edges
+ *                   (graph 0 of hz_bfs_batch_create with this seed), built on
+ *                   the device, then BFS from vertex 0; output 0: i32 levels,
+ *                   1: row_ptr, 2: col.
+ * Times are milliseconds on the server's steady clock. */
+#define HZ_JOB_COLLATZ 0u
+#define HZ_JOB_PHOTON 1u
+#define HZ_JOB_BFS 2u
+#define HZ_JOB_PENDING 0
+#define HZ_JOB_RUNNING 1
+#define HZ_JOB_DONE 2
+#define HZ_JOB_FAILED (-1)
+
+typedef struct hz_server hz_server;
+typedef struct hz_server_config {
+    uint64_t pool_bytes;  /* device memory for job state (0 = 8 GiB) */
+    uint64_t arena_links; /* per slot and program (0 = 2^16; BFS gets 4x) */
+    uint32_t slots;       /* concurrent jobs, 1..8 (0 = 4, the hardware queues of a process) */
+    uint32_t max_pending; /* submissions beyond this many waiting jobs fail (0 = no bound) */
+    uint32_t keep_outputs; /* 1: a finished job keeps its state until hz_server_release */
+    uint32_t reserved;
+} hz_server_config;
+typedef struct hz_job {
+    uint32_t type; /* HZ_JOB_* */
+    uint32_t size; /* work items: starts, photons or vertices */
+    uint64_t seed;
+} hz_job;
+typedef struct hz_job_result {
+    uint64_t id;
+    uint32_t type;
+    uint32_t size;
+    uint64_t seed;
+    int32_t status; /* HZ_JOB_PENDING, _RUNNING, _DONE or _FAILED */
+    uint32_t slot;
+    uint64_t promises;
+    uint32_t launches;
+    uint32_t groups;
+    double submit_ms; /* arrival */
+    double start_ms;  /* admission (launch) */
+    double end_ms;    /* completion seen by the server */
+    uint64_t state_bytes;
+} hz_job_result;
+typedef struct hz_server_info_t {
+    uint32_t slots;
+    uint32_t pending;
+    uint64_t jobs;
+    uint64_t pool_bytes;
+    uint64_t pool_used;
+    uint64_t pool_largest_free;
+    uint32_t groups_collatz; /* groups a job may use, per program */
+    uint32_t groups_photon;
+    uint32_t groups_bfs;
+    uint32_t reserved;
+} hz_server_info_t;
+
+int hz_server_create(const hz_server_config* config, hz_server** out);
+int hz_server_destroy(hz_server* server);
+/* Queues a job (stamped with the current time) and makes progress. */
+int hz_server_submit(hz_server* server, const hz_job* job, uint64_t* id);
+/* Non-blocking: reaps finished passes, admits pending jobs. */
+int hz_server_progress(hz_server* server, uint32_t* running, uint32_t* pending);
+/* Makes progress until no job is pending or running. */
+int hz_server_drain(hz_server* server, double timeout_ms);
+int hz_server_job(hz_server* server, uint64_t id, hz_job_result* result);
+/* Copies output `which` of a finished job (kept with keep_outputs); `bytes`
+ * must equal the output's size. */
+int hz_server_output(hz_server* server, uint64_t id, uint32_t which, void* host, uint64_t bytes);
+int hz_server_release(hz_server* server, uint64_t id);
+int hz_server_info(const hz_server* server, hz_server_info_t* info);
+/* Open-loop replay: job i arrives arrival_ms[i] (sorted) after the call
+ * starts; returns when all are finished. results[i] describes job i. */
+int hz_server_run_trace(hz_server* server, const hz_job* jobs, const double* arrival_ms, uint32_t count,
+                        double timeout_ms, hz_job_result* results);
+/* Host-only check of the job-state pool allocator: ops[i] >= 0 allocates
+ * that many bytes (out[i] = offset, or all-ones if it does not fit);
+ * ops[i] = -k - 1 frees the k-th allocation (out[i] = largest free block). */
+int hz_pool_test(uint64_t size, const int64_t* ops, uint32_t count, uint64_t* out);
+
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {
     int32_t device;
