@@ -167,6 +167,71 @@ def traversed_edges(hz, batch, reached) -> int:
     return total
 
 
+SERVE_JOBS = 300
+
+
+def serving_mix(n: int, seed: int) -> list:
+    """Jobs drawn as the serving workload defines them: Collatz, photon walk or
+    BFS with probability 1/3 each, size log-uniform in [2^10, 2^22] work
+    items, a seed per job."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice(["collatz", "photon", "bfs"], size=n)
+    sizes = np.rint(2.0 ** rng.uniform(10.0, 22.0, size=n)).astype(np.int64)
+    seeds = rng.integers(1, 1 << 40, size=n)
+    return [(str(k), int(s), int(sd)) for k, s, sd in zip(kinds, sizes, seeds)]
+
+
+def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -> dict:
+    """Peak throughput with every job arriving at once, then open-loop Poisson
+    arrivals at fractions of that peak: p50/p99 latency (arrival to completion
+    seen by the server) and achieved throughput at each load."""
+    srv = hz.Server(pool_bytes=48 << 30, slots=4)
+    try:
+        warm = serving_mix(12, 99)
+        srv.run_trace(warm, np.zeros(len(warm)))
+        mix = serving_mix(jobs, 1)
+        res = srv.run_trace(mix, np.zeros(len(mix)))
+        span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
+        peak = len(res) / (span * 1e-3)
+        promises = sum(r["promises"] for r in res)
+        out = {
+            "workload": f"{jobs} jobs, Collatz / photon walk / BFS with p = 1/3 each, sizes log-uniform in "
+                        f"[2^10, 2^22] items, per-job seeds; 4 slots (concurrent streams)",
+            "peak_jobs_per_s": peak,
+            "peak_promises_per_s": promises / (span * 1e-3),
+            "peak_makespan_ms": span,
+            "loads": [],
+        }
+        rng = np.random.default_rng(2)
+        for k, load in enumerate(loads):
+            mix = serving_mix(jobs, 10 + k)
+            rate = load * peak  # jobs per second
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(mix))) * 1e3
+            arrivals -= arrivals[0]
+            res = srv.run_trace(mix, arrivals)
+            lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+            span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
+            by_kind = {}
+            for kind in ("collatz", "photon", "bfs"):
+                sel = np.array([m[0] == kind for m in mix])
+                if sel.any():
+                    by_kind[kind] = {"p50_ms": float(np.percentile(lat[sel], 50)),
+                                     "p99_ms": float(np.percentile(lat[sel], 99))}
+            out["loads"].append({
+                "offered_fraction_of_peak": load,
+                "offered_jobs_per_s": rate,
+                "achieved_jobs_per_s": len(res) / (span * 1e-3),
+                "achieved_promises_per_s": sum(r["promises"] for r in res) / (span * 1e-3),
+                "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)),
+                "max_ms": float(lat.max()),
+                "by_kind": by_kind,
+            })
+        return out
+    finally:
+        srv.close()
+
+
 def _leg_collatz(kind):
     def run(hz):
         starts = hz.collatz_make_starts(SEED, BITS, COUNT)
@@ -225,6 +290,7 @@ def main() -> int:
     ap.add_argument("--no-event", action="store_true", help="skip the EventProgram comparator")
     ap.add_argument("--no-photon", action="store_true", help="skip the photon-walk Async vs Event comparison")
     ap.add_argument("--no-bfs", action="store_true", help="skip the batched BFS Async vs Event comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving load sweep")
     ap.add_argument("--leg", choices=sorted(LEGS), help="run one leg once (for counter collection) and exit")
     args = ap.parse_args()
 
@@ -321,6 +387,9 @@ def main() -> int:
 
     if not args.no_bfs:
         out["bfs"] = bfs_comparison(hz)
+
+    if not args.no_serve:
+        out["serving"] = serving_sweep(hz)
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
     out["plain_kernel"] = {

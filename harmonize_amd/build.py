@@ -97,6 +97,19 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+REFERENCE = Path(os.environ.get("HZ_REFERENCE", "/root/reference"))
+
+
+def build_reference_examples() -> Path | None:
+    """Builds the reference's own Collatz examples against this runtime's
+    headers into oracle/_ref/ (oracle/build_ref.sh) when the reference
+    sources are present; returns the output folder, or None."""
+    if not (REFERENCE / "examples" / "cpp" / "collaz" / "collaz_hrm.cpp").exists():
+        return None
+    _run(["bash", str(ORACLE_DIR / "build_ref.sh")])
+    return ORACLE_DIR / "_ref"
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--runtime", action="store_true", help="build only the runtime library")
@@ -108,6 +121,9 @@ def main(argv: list[str] | None = None) -> int:
         print("built", build_runtime(args.jobs))
     if args.oracle or both:
         print("built", build_oracle())
+        ref = build_reference_examples()
+        if ref:
+            print("built", ref)
     return 0
 
 
