@@ -197,6 +197,13 @@ class AsyncProgram {
     // safe; the wavefronts that still hold work finish it, and completion is
     // decided by the last one out.
     static constexpr uint32_t IDLE_EXIT_POLLS = 1u << 16;
+#ifndef HZ_IDLE_BACKOFF_MAX
+#define HZ_IDLE_BACKOFF_MAX 1
+#endif
+    // Longest idle pause, in units of 256 shader cycles. Growing pauses
+    // (e.g. 32) cut idle polling traffic but cost Collatz 3% (groups pick up
+    // freshly queued links later), so the default pause stays fixed.
+    static constexpr uint32_t IDLE_BACKOFF_MAX = HZ_IDLE_BACKOFF_MAX;
 
     struct ThreadContext {
         unsigned int thread_id;
@@ -899,12 +906,11 @@ class AsyncProgram {
         units = hz::wave::first(units);
         flags = hz::wave::first(flags);
         if ((makers == 0 && units == 0) || (flags & ~COMPLETION_FLAG) != 0) return DONE;
-        // Back off while idle: each poll reads the queue and status lines
-        // that busy groups update, so thousands of idle wavefronts polling at
-        // full rate slow everyone's atomics. The pause grows with the idle
-        // streak up to about 8K shader cycles.
+        // Pause between idle polls; each poll reads queue and status lines
+        // that busy groups update. The pause may grow with the idle streak
+        // (IDLE_BACKOFF_MAX).
         uint32_t pauses = streak / 16u + 1u;
-        if (pauses > 32u) pauses = 32u;
+        if (pauses > IDLE_BACKOFF_MAX) pauses = IDLE_BACKOFF_MAX;
         for (uint32_t i = 0; i < pauses; ++i) __builtin_amdgcn_s_sleep(4);
         return RUN;
     }
