@@ -555,10 +555,12 @@ class JobResult(ctypes.Structure):
         ("start_ms", ctypes.c_double),
         ("end_ms", ctypes.c_double),
         ("state_bytes", ctypes.c_uint64),
+        ("chunks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class ServerInfo(ctypes.Structure):
@@ -647,7 +649,7 @@ class Server:
         info = self.job(i)
         dtype, per = self.OUTPUTS[info["type"]][which]
         if info["type"] == JOB_PHOTON and which == 1:
-            count = 64 * (1000 + PHOTON_EXTRA_SLOTS)
+            count = info["chunks"] * 64 * (1000 + PHOTON_EXTRA_SLOTS)
         elif info["type"] == JOB_BFS and which == 1:
             count = info["size"] + 1
         else:

@@ -18,6 +18,19 @@ static __global__ void __launch_bounds__(256)
     }
 }
 
+// Starts first .. first + count - 1 of the same sequence.
+static __global__ void __launch_bounds__(256) collatz_starts_range_kernel(unsigned long long seed, unsigned int bits,
+                                                                           unsigned int first, unsigned int count,
+                                                                           unsigned long long* out) {
+    const unsigned long long modulus = (1ull << bits) - 1ull;
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const unsigned int i = first + t;
+        hz::philox::Ctr r = hz::philox::philox4x32_10(i, 0u, 0u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        unsigned long long x = ((unsigned long long)r.v[1] << 32) | r.v[0];
+        out[i] = x % modulus + 1ull;
+    }
+}
+
 }  // namespace collatz
 }  // namespace hz
 

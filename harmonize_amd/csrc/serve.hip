@@ -2,10 +2,14 @@
 // and BFS jobs as they arrive and runs them concurrently on one GPU.
 //
 // Continuous batching, gfx950-first:
-//  - the device is shared by `slots` concurrent jobs, each on its own
-//    non-blocking HIP stream (one hardware queue each); a job's persistent
-//    AsyncProgram kernel is launched with the slot's share of the resident
-//    wavefronts, so the jobs of different slots run side by side;
+//  - the device is shared by `slots` concurrent units of work, each on its
+//    own non-blocking HIP stream (one hardware queue each); a unit's
+//    persistent AsyncProgram kernel is launched with the slot's share of the
+//    resident wavefronts, so the units of different slots run side by side;
+//  - a large Collatz or photon job is cut into up to `slots` chunks of
+//    contiguous work items (ids stay global, so the results are those of the
+//    whole job) that run on different slots; a BFS job is one unit and runs
+//    under the level-synchronous EventProgram, which suits a single search;
 //  - each slot keeps its program instances (arena, queues, status) resident
 //    from server creation, so admitting a job allocates nothing on the device;
 //  - job state (inputs, outputs, graph, sort scratch) lives in one device pool
@@ -37,12 +41,15 @@ namespace {
 
 using SCollatz = AsyncProgram<CollatzSpec>;
 using SPhoton = AsyncProgram<PhotonSpec>;
-using SBfs = AsyncProgram<BfsSpecT<1>>;
+using SBfs = EventProgram<BfsSpecT<1>>;
 
 constexpr uint32_t COLLATZ_BITS = 32;      // serving Collatz starts are uniform in [1, 2^32)
 constexpr uint32_t PHOTON_REPLICAS = 64;   // tally copies per photon job
 constexpr uint32_t BFS_EDGE_FACTOR = 16;
 constexpr uint64_t ALIGN = 256;
+constexpr uint32_t CHUNK_MIN_ITEMS = 1u << 18;  // smallest chunk a job is cut into
+constexpr uint64_t BFS_EVENT_IO = 1ull << 23;    // per-operation queue capacity of a slot's BFS program
+constexpr uint32_t BFS_GENERATIONS_PER_PASS = 4; // generations enqueued between completion checks
 
 uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -128,7 +135,7 @@ size_t bfs_sort_temp(uint32_t edges) {
     return temp;
 }
 
-Layout layout_for(const hz_job& j) {
+Layout layout_for(const hz_job& j, uint32_t chunks) {
     Layout L;
     uint64_t n = j.size;
     auto take = [&](uint64_t bytes) {
@@ -143,7 +150,7 @@ Layout layout_for(const hz_job& j) {
             break;
         case HZ_JOB_PHOTON:
             L.a = take(4 * n);  // events
-            L.b = take((uint64_t)PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8);  // tally
+            L.b = take((uint64_t)chunks * PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8);  // tally per chunk
             break;
         case HZ_JOB_BFS: {
             uint64_t E = n * BFS_EDGE_FACTOR;
@@ -163,9 +170,9 @@ Layout layout_for(const hz_job& j) {
     return L;
 }
 
-__global__ void set_iter_kernel(util::iter::AtomicIter<unsigned int>* it, unsigned int limit) {
+__global__ void set_iter_kernel(util::iter::AtomicIter<unsigned int>* it, unsigned int first, unsigned int limit) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        it->value = 0u;
+        it->value = first;
         it->limit = limit;
     }
 }
@@ -196,8 +203,17 @@ struct Job {
     double submit_ms = 0, start_ms = 0, end_ms = 0;
     uint64_t promises = 0;
     uint32_t launches = 0, groups = 0;
+    uint32_t chunks = 1, chunks_done = 0, chunks_started = 0;
     std::string error;
+
+    uint64_t first(uint32_t c) const { return (uint64_t)desc.size * c / chunks; }
+    uint64_t end(uint32_t c) const { return (uint64_t)desc.size * (c + 1) / chunks; }
 };
+
+uint32_t event_share(uint32_t slots) {
+    uint32_t g = (uint32_t)hz::capi::compute_units() * 32u / slots;
+    return g < 4 ? 4 : g;
+}
 
 template <typename P>
 uint32_t share_of(uint32_t slots) {
@@ -215,18 +231,20 @@ struct hz_server {
     void* pool_base = nullptr;
     std::unique_ptr<Pool> pool;
     std::deque<Job> jobs;  // by id
-    std::deque<uint64_t> pending;
+    std::deque<std::pair<uint64_t, uint32_t>> pending;  // (job, chunk) in arrival order
     uint32_t groups_collatz = 0, groups_photon = 0, groups_bfs = 0;
 
     struct Slot {
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
-        hz::AsyncStatus* status = nullptr;  // pinned host copy
+        hz::AsyncStatus* status = nullptr;      // pinned host copy (async programs)
+        SBfs::Status* event_status = nullptr;   // pinned host copy (the BFS event program)
         hz::capi::DeviceIter iter{0};
         std::unique_ptr<SCollatz::Instance> collatz;
         std::unique_ptr<SPhoton::Instance> photon;
         std::unique_ptr<SBfs::Instance> bfs;
         int64_t job = -1;
+        uint32_t chunk = 0;
     };
     std::vector<std::unique_ptr<Slot>> slots;
 
@@ -238,6 +256,7 @@ struct hz_server {
             if (s->done) (void)hipEventDestroy(s->done);
             if (s->stream) (void)hipStreamDestroy(s->stream);
             if (s->status) (void)hipHostFree(s->status);
+            if (s->event_status) (void)hipHostFree(s->event_status);
         }
         slots.clear();
         if (pool_base) (void)hipFree(pool_base);
@@ -261,42 +280,61 @@ struct hz_server {
         check(hipEventRecord(s.done, s.stream), "serve event");
     }
 
-    void launch(Slot& s, Job& j) {
-        const uint64_t n = j.desc.size;
+    // BFS_GENERATIONS_PER_PASS generations of the slot's BFS program, then the status copy and the event.
+    void enqueue_generations(Slot& s, uint32_t groups) {
+        for (uint32_t g = 0; g < BFS_GENERATIONS_PER_PASS; ++g) {
+            exec<SBfs>(*s.bfs, groups, 4u, s.stream);
+            check(hipGetLastError(), "serve generation launch");
+        }
+        check(hipMemcpyAsync(s.event_status, s.bfs->status.data(), sizeof(SBfs::Status), hipMemcpyDeviceToHost,
+                             s.stream),
+              "serve status copy");
+        check(hipEventRecord(s.done, s.stream), "serve event");
+    }
+
+    // Enqueues chunk c of job j on slot s.
+    void launch(Slot& s, Job& j, uint32_t c) {
+        const uint64_t first = j.first(c), end = j.end(c), count = end - first;
         switch (j.desc.type) {
             case HZ_JOB_COLLATZ: {
                 auto* starts = (unsigned long long*)mem(j, j.layout.a);
                 auto* steps = (unsigned int*)mem(j, j.layout.b);
-                hipLaunchKernelGGL(hz::collatz::collatz_starts_kernel, dim3(hz::graph::grid_for(n)), dim3(256), 0,
-                                   s.stream, (unsigned long long)j.desc.seed, COLLATZ_BITS, (unsigned int)n, starts);
-                hipLaunchKernelGGL(set_iter_kernel, dim3(1), dim3(64), 0, s.stream, s.iter.ptr, (unsigned int)n);
+                hipLaunchKernelGGL(hz::collatz::collatz_starts_range_kernel, dim3(hz::graph::grid_for(count)),
+                                   dim3(256), 0, s.stream, (unsigned long long)j.desc.seed, COLLATZ_BITS,
+                                   (unsigned int)first, (unsigned int)count, starts);
+                hipLaunchKernelGGL(set_iter_kernel, dim3(1), dim3(64), 0, s.stream, s.iter.ptr, (unsigned int)first,
+                                   (unsigned int)end);
                 s.collatz->device_state = CollatzState{starts, steps, s.iter.ptr};
-                j.groups = job_groups(groups_collatz, n);
-                init<SCollatz>(*s.collatz, j.groups, s.stream);
-                enqueue_exec<SCollatz>(s, *s.collatz, j.groups);
+                uint32_t groups = job_groups(groups_collatz, count);
+                j.groups = groups > j.groups ? groups : j.groups;
+                init<SCollatz>(*s.collatz, groups, s.stream);
+                enqueue_exec<SCollatz>(s, *s.collatz, groups);
                 break;
             }
             case HZ_JOB_PHOTON: {
                 hz_photon_params p{};
                 hz_photon_defaults(&p);
-                p.photons = (uint32_t)n;
+                p.photons = j.desc.size;
                 p.seed = j.desc.seed;
                 p.replicas = PHOTON_REPLICAS;
                 auto* events = (uint32_t*)mem(j, j.layout.a);
-                auto* tally = (double*)mem(j, j.layout.b);
-                uint64_t tally_words = (uint64_t)PHOTON_REPLICAS * (p.bins + HZ_PHOTON_EXTRA_SLOTS) * 2;
-                hipLaunchKernelGGL(zero_kernel, dim3(hz::graph::grid_for(n)), dim3(256), 0, s.stream, events, n);
-                hipLaunchKernelGGL(zero_kernel, dim3(hz::graph::grid_for(tally_words)), dim3(256), 0, s.stream,
-                                   (uint32_t*)tally, tally_words);
-                hipLaunchKernelGGL(set_iter_kernel, dim3(1), dim3(64), 0, s.stream, s.iter.ptr, (unsigned int)n);
+                const uint64_t tally_doubles = (uint64_t)PHOTON_REPLICAS * (p.bins + HZ_PHOTON_EXTRA_SLOTS);
+                auto* tally = (double*)mem(j, j.layout.b) + tally_doubles * c;
+                hipLaunchKernelGGL(zero_kernel, dim3(hz::graph::grid_for(count)), dim3(256), 0, s.stream,
+                                   events + first, count);
+                hipLaunchKernelGGL(zero_kernel, dim3(hz::graph::grid_for(2 * tally_doubles)), dim3(256), 0,
+                                   s.stream, (uint32_t*)tally, 2 * tally_doubles);
+                hipLaunchKernelGGL(set_iter_kernel, dim3(1), dim3(64), 0, s.stream, s.iter.ptr, (unsigned int)first,
+                                   (unsigned int)end);
                 s.photon->device_state = hz::photon_abi::make_state(p, tally, events, s.iter.ptr);
-                j.groups = job_groups(groups_photon, n);
-                init<SPhoton>(*s.photon, j.groups, s.stream);
-                enqueue_exec<SPhoton>(s, *s.photon, j.groups);
+                uint32_t groups = job_groups(groups_photon, count);
+                j.groups = groups > j.groups ? groups : j.groups;
+                init<SPhoton>(*s.photon, groups, s.stream);
+                enqueue_exec<SPhoton>(s, *s.photon, groups);
                 break;
             }
             case HZ_JOB_BFS: {
-                const uint32_t V = (uint32_t)n;
+                const uint32_t V = j.desc.size;
                 const uint32_t E = V * BFS_EDGE_FACTOR;
                 const uint32_t scale = hz::rmat::scale_of(V);
                 auto* row_ptr = (int32_t*)mem(j, j.layout.a);
@@ -325,17 +363,30 @@ struct hz_server {
                 st.job_count = 1;
                 st.split = BFS_DEFAULT_SPLIT;
                 s.bfs->device_state = st;
-                j.groups = job_groups(groups_bfs, n);
-                init<SBfs>(*s.bfs, j.groups, s.stream);
-                enqueue_exec<SBfs>(s, *s.bfs, j.groups);
+                s.bfs->reset(s.stream);
+                j.groups = job_groups(groups_bfs, V);
+                enqueue_generations(s, j.groups);
                 break;
             }
         }
-        j.launches = 1;
+        j.launches++;
     }
 
-    // The pass that just finished: 1 = job done, 0 = relaunched, throws on a runtime error.
+    // The pass that just finished: 1 = chunk done, 0 = more work enqueued,
+    // throws on a runtime error.
     int finish_pass(Slot& s, Job& j) {
+        if (j.launches >= 1000000u) throw std::runtime_error("serve: launch bound reached");
+        if (j.desc.type == HZ_JOB_BFS) {
+            const SBfs::Status& st = *s.event_status;
+            if (st.flags & SBfs::OVERFLOW_FLAG) throw std::runtime_error("serve: a BFS generation overflowed its queue");
+            if (st.flags & SBfs::COMPLETION_FLAG) {
+                j.promises += st.promises;
+                return 1;
+            }
+            j.launches++;
+            enqueue_generations(s, j.groups);
+            return 0;
+        }
         const hz::AsyncStatus& st = *s.status;
         if (st.flags & SCollatz::ARENA_EXHAUSTED_FLAG) {
             throw std::runtime_error("serve: the slot's link arena was exhausted (raise arena_links)");
@@ -344,20 +395,16 @@ struct hz_server {
             throw std::runtime_error(std::string("serve: a bounded wait gave up (") + hz::stall_name(st.stall_site) +
                                      ")");
         }
-        j.promises = st.promises;
-        if (st.flags & SCollatz::COMPLETION_FLAG) return 1;
-        if (j.launches >= 1000000u) throw std::runtime_error("serve: launch bound reached");
+        if (st.flags & SCollatz::COMPLETION_FLAG) {
+            j.promises += st.promises;
+            return 1;
+        }
         j.launches++;
-        switch (j.desc.type) {
-            case HZ_JOB_COLLATZ:
-                enqueue_exec<SCollatz>(s, *s.collatz, j.groups);
-                break;
-            case HZ_JOB_PHOTON:
-                enqueue_exec<SPhoton>(s, *s.photon, j.groups);
-                break;
-            default:
-                enqueue_exec<SBfs>(s, *s.bfs, j.groups);
-                break;
+        const uint32_t groups = j.groups;
+        if (j.desc.type == HZ_JOB_COLLATZ) {
+            enqueue_exec<SCollatz>(s, *s.collatz, groups);
+        } else {
+            enqueue_exec<SPhoton>(s, *s.photon, groups);
         }
         return 0;
     }
@@ -369,7 +416,7 @@ struct hz_server {
         }
     }
 
-    // Reaps finished passes and admits pending jobs into free slots.
+    // Reaps finished passes and admits pending chunks into free slots.
     void progress() {
         for (auto& sp : slots) {
             Slot& s = *sp;
@@ -382,40 +429,57 @@ struct hz_server {
             try {
                 done = finish_pass(s, j);
             } catch (const std::exception& e) {
-                j.status = HZ_JOB_FAILED;
-                j.error = e.what();
-                j.end_ms = now_ms(t0);
-                done = -1;  // the next job's init clears the instance's flags
+                if (j.status != HZ_JOB_FAILED) {
+                    j.status = HZ_JOB_FAILED;
+                    j.error = e.what();
+                    j.end_ms = now_ms(t0);
+                }
+                done = -1;  // the next unit's init clears the instance's flags
             }
             if (done == 0) continue;
-            if (done == 1) {
-                j.status = HZ_JOB_DONE;
-                j.end_ms = now_ms(t0);
-            }
-            if (!cfg.keep_outputs || j.status == HZ_JOB_FAILED) release_memory(j);
             s.job = -1;
+            j.chunks_done++;
+            if (j.chunks_done == j.chunks_started && (j.chunks_done == j.chunks || j.status == HZ_JOB_FAILED)) {
+                if (j.status != HZ_JOB_FAILED) {
+                    j.status = HZ_JOB_DONE;
+                    j.end_ms = now_ms(t0);
+                }
+                if (!cfg.keep_outputs || j.status == HZ_JOB_FAILED) release_memory(j);
+            }
         }
         for (auto& sp : slots) {
-            if (pending.empty()) break;
             Slot& s = *sp;
             if (s.job >= 0) continue;
-            Job& j = jobs[(size_t)pending.front()];
-            uint64_t off = pool->alloc(j.layout.bytes);
-            if (off == Pool::NONE) break;  // the head waits for memory; arrival order is kept
-            pending.pop_front();
-            j.off = off;
-            j.slot = (uint32_t)(&sp - &slots[0]);
-            j.status = HZ_JOB_RUNNING;
-            j.start_ms = now_ms(t0);
-            s.job = (int64_t)j.id;
-            try {
-                launch(s, j);
-            } catch (...) {
-                j.status = HZ_JOB_FAILED;
-                j.end_ms = now_ms(t0);
-                release_memory(j);
-                s.job = -1;
-                throw;
+            while (!pending.empty()) {
+                auto [id, c] = pending.front();
+                Job& j = jobs[(size_t)id];
+                if (j.status == HZ_JOB_FAILED) {  // drop the rest of a failed job
+                    pending.pop_front();
+                    continue;
+                }
+                if (j.off == Pool::NONE) {
+                    uint64_t off = pool->alloc(j.layout.bytes);
+                    if (off == Pool::NONE) return;  // the head waits for memory; arrival order is kept
+                    j.off = off;
+                    j.status = HZ_JOB_RUNNING;
+                    j.start_ms = now_ms(t0);
+                    j.slot = (uint32_t)(&sp - &slots[0]);
+                }
+                pending.pop_front();
+                s.job = (int64_t)j.id;
+                s.chunk = c;
+                j.chunks_started++;
+                try {
+                    launch(s, j, c);
+                } catch (...) {
+                    j.status = HZ_JOB_FAILED;
+                    j.end_ms = now_ms(t0);
+                    s.job = -1;
+                    j.chunks_done++;
+                    if (j.chunks_done == j.chunks_started) release_memory(j);
+                    throw;
+                }
+                break;
             }
         }
     }
@@ -429,14 +493,19 @@ struct hz_server {
         Job j;
         j.id = jobs.size();
         j.desc = d;
-        j.layout = layout_for(d);
+        if (d.type != HZ_JOB_BFS) {
+            uint64_t k = (d.size + CHUNK_MIN_ITEMS - 1) / CHUNK_MIN_ITEMS;
+            j.chunks = (uint32_t)(k < cfg.slots ? k : cfg.slots);
+            if (j.chunks == 0) j.chunks = 1;
+        }
+        j.layout = layout_for(d, j.chunks);
         if (j.layout.bytes > pool->size()) throw std::invalid_argument("hz_server_submit: job larger than the pool");
         j.submit_ms = submit_ms;
         if (cfg.max_pending && pending.size() >= cfg.max_pending) {
             throw std::runtime_error("hz_server_submit: too many pending jobs");
         }
         jobs.push_back(j);
-        pending.push_back(j.id);
+        for (uint32_t c = 0; c < j.chunks; ++c) pending.emplace_back(j.id, c);
         return j.id;
     }
 
@@ -463,6 +532,7 @@ struct hz_server {
         r->start_ms = j.start_ms;
         r->end_ms = j.end_ms;
         r->state_bytes = j.layout.bytes;
+        r->chunks = j.chunks;
     }
 
     Job& job(uint64_t id) {
@@ -493,16 +563,19 @@ extern "C" int hz_server_create(const hz_server_config* config, hz_server** out)
         s->pool = std::make_unique<Pool>(c.pool_bytes);
         s->groups_collatz = share_of<SCollatz>(c.slots);
         s->groups_photon = share_of<SPhoton>(c.slots);
-        s->groups_bfs = share_of<SBfs>(c.slots);
+        s->groups_bfs = event_share(c.slots);
         for (uint32_t i = 0; i < c.slots; ++i) {
             auto sl = std::make_unique<hz_server::Slot>();
             check(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking), "hipStreamCreate");
             check(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming), "hipEventCreate");
             check(hipHostMalloc((void**)&sl->status, sizeof(hz::AsyncStatus), hipHostMallocDefault), "hipHostMalloc");
             std::memset(sl->status, 0, sizeof(hz::AsyncStatus));
+            check(hipHostMalloc((void**)&sl->event_status, sizeof(SBfs::Status), hipHostMallocDefault),
+                  "hipHostMalloc");
+            std::memset(sl->event_status, 0, sizeof(SBfs::Status));
             sl->collatz = std::make_unique<SCollatz::Instance>(c.arena_links, CollatzState{}, 0, 64u);
             sl->photon = std::make_unique<SPhoton::Instance>(c.arena_links, PhotonState{}, 0, 64u);
-            sl->bfs = std::make_unique<SBfs::Instance>(c.arena_links * 4, BfsState{}, 0, 64u);
+            sl->bfs = std::make_unique<SBfs::Instance>(BFS_EVENT_IO, BfsState{});
             s->slots.push_back(std::move(sl));
         }
         check(hipDeviceSynchronize(), "serve setup");
@@ -571,7 +644,7 @@ extern "C" int hz_server_output(hz_server* server, uint64_t id, uint32_t which, 
                 break;
             case HZ_JOB_PHOTON:
                 if (which == 0) sub = j.layout.a, want = 4 * n;          // events
-                else if (which == 1) sub = j.layout.b, want = (uint64_t)PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8;
+                else if (which == 1) sub = j.layout.b, want = (uint64_t)j.chunks * PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8;
                 break;
             case HZ_JOB_BFS:
                 if (which == 0) sub = j.layout.c, want = 4 * n;          // levels

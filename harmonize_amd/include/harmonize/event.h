@@ -181,6 +181,11 @@ class EventProgram {
                 hipMemcpy(&status.data()->flags, &zero, sizeof(unsigned int), hipMemcpyHostToDevice));
         }
 
+        // Clears every queue counter, the flags and the statistics with a
+        // kernel on `stream`, so that the instance can run a new job without
+        // a synchronous copy (queued promises are discarded).
+        void reset(hipStream_t stream);
+
         unsigned long long int promises() { return read_status().promises; }
     };
 
@@ -346,6 +351,15 @@ class EventProgram {
     // Launched as one 64-lane group after each generation; lane s handles
     // segment s of every operation.
     static_assert(SEGMENTS == 64, "flip assigns one segment per lane");
+    // Zeroes the Status struct (grid-stride over its 32-bit words).
+    __device__ static void kernel_reset(DeviceContext ctx) {
+        unsigned int* w = (unsigned int*)ctx.status;
+        const size_t words = sizeof(Status) / sizeof(unsigned int);
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x) {
+            w[i] = 0u;
+        }
+    }
+
     __device__ static void flip(DeviceContext ctx) {
         Status* st = ctx.status;
         const uint32_t lane = hz::wave::lane();

@@ -24,6 +24,18 @@ __global__ void __launch_bounds__(64) _hz_event_flip_kernel(typename ProgType::D
     ProgType::flip(ctx);
 }
 
+template <typename ProgType>
+__global__ void __launch_bounds__(256) _hz_event_reset_kernel(typename ProgType::DeviceContext ctx) {
+    ProgType::kernel_reset(ctx);
+}
+
+template <typename PROGRAM_SPEC>
+void EventProgram<PROGRAM_SPEC>::Instance::reset(hipStream_t stream) {
+    hipLaunchKernelGGL(_hz_event_reset_kernel<EventProgram<PROGRAM_SPEC>>, dim3(16), dim3(256), 0, stream,
+                       to_context());
+    util::host::auto_throw(hipGetLastError());
+}
+
 template <typename PROGRAM_SPEC>
 void AsyncProgram<PROGRAM_SPEC>::Instance::reset(hipStream_t stream) {
     size_t slots = (size_t)queue_capacity * (queue_count + 1u);

@@ -327,18 +327,20 @@ int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tu
                       hz_tune_trial* log, uint32_t log_capacity);
 
 /* ------------------------------------------------------------- serving
- * A request server for Collatz, photon-walk and BFS jobs. Up to `slots` jobs
- * run at once, each on its own stream with the slot's share of the device;
- * the program instances of every slot are created with the server, so
- * admission allocates nothing. Job state lives in a device pool; pending
- * jobs are admitted in arrival order when a slot is free and their state
- * fits. Jobs:
+ * A request server for Collatz, photon-walk and BFS jobs. Up to `slots` units
+ * of work run at once, each on its own stream with the slot's share of the
+ * device; the program instances of every slot are created with the server,
+ * so admission allocates nothing. A Collatz or photon job of more than 2^18
+ * items is cut into up to `slots` chunks of contiguous items; a BFS job runs
+ * as one unit under the EventProgram. Job state lives in a device pool; a
+ * job's state is allocated when its first unit is admitted, in arrival
+ * order, once a slot is free and the state fits. Jobs:
  *   HZ_JOB_COLLATZ  size starts from Philox4x32-10 keyed by seed, uniform in
  *                   [1, 2^32) (hz_collatz_make_starts with bits = 32);
  *                   output 0: u32 steps, output 1: u64 starts.
  *   HZ_JOB_PHOTON   the photon walk with hz_photon_defaults except photons =
  *                   size, seed = seed, replicas = 64; output 0: u32 events per
- *                   photon, output 1: the 64 tally replicas (doubles).
+ *                   photon, output 1: the tally replicas (doubles), 64 per chunk.
  *   HZ_JOB_BFS      an R-MAT graph of size vertices and 16 x size edges
  *                   (graph 0 of hz_bfs_batch_create with this seed), built on
  *                   the device, then BFS from vertex 0; output 0: i32 levels,
@@ -355,7 +357,7 @@ int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tu
 typedef struct hz_server hz_server;
 typedef struct hz_server_config {
     uint64_t pool_bytes;  /* device memory for job state (0 = 8 GiB) */
-    uint64_t arena_links; /* per slot and program (0 = 2^16; BFS gets 4x) */
+    uint64_t arena_links; /* per slot and async program (0 = 2^16) */
     uint32_t slots;       /* concurrent jobs, 1..8 (0 = 4, the hardware queues of a process) */
     uint32_t max_pending; /* submissions beyond this many waiting jobs fail (0 = no bound) */
     uint32_t keep_outputs; /* 1: a finished job keeps its state until hz_server_release */
@@ -380,6 +382,8 @@ typedef struct hz_job_result {
     double start_ms;  /* admission (launch) */
     double end_ms;    /* completion seen by the server */
     uint64_t state_bytes;
+    uint32_t chunks;  /* units the job was cut into (Collatz, photon) */
+    uint32_t reserved;
 } hz_job_result;
 typedef struct hz_server_info_t {
     uint32_t slots;

@@ -21,7 +21,7 @@ def hz():
 
 
 def tally_sum(replicas):
-    return replicas.reshape(64, -1).sum(axis=0)
+    return replicas.reshape(-1, 1005).sum(axis=0)
 
 
 def assert_close(a, b):
@@ -49,7 +49,8 @@ def check_against_oracle(hz, srv, i, kind, size, seed):
         assert np.array_equal(srv.output(i, 0), O.bfs_levels([size], e, r, c))
 
 
-@pytest.mark.parametrize("kind,size", [("collatz", 1), ("collatz", 5000), ("photon", 1), ("photon", 20000),
+@pytest.mark.parametrize("kind,size", [("collatz", 1), ("collatz", 5000), ("collatz", (1 << 20) + 7),
+                                       ("photon", 1), ("photon", 20000), ("photon", (1 << 19) + 3),
                                        ("bfs", 1), ("bfs", 3000)])
 def test_job_matches_oracle(hz, kind, size):
     srv = hz.Server(pool_bytes=1 << 30, slots=2, keep_outputs=True)
