@@ -36,6 +36,7 @@ ARENA_PROMISES = 1 << 24
 LINK_PROMISES = 64
 HBM_PEAK_GBPS = 8000.0
 PROMISE_BYTES = 16  # packed Collatz promise {u64 value, u32 step, u32 index}
+QUEUES = 16  # link queues of the async program (the tuner's choice; 8 is 1% slower)
 
 
 def algorithmic_bytes(promises: int, starts: int) -> int:
@@ -236,7 +237,7 @@ def _leg_collatz(kind):
     def run(hz):
         starts = hz.collatz_make_starts(SEED, BITS, COUNT)
         steps_out = hz.DeviceArray(COUNT, np.uint32)
-        cfg = dict(arena=ARENA_PROMISES // LINK_PROMISES) if kind == "async" else dict(max_launches=1 << 20)
+        cfg = dict(arena=ARENA_PROMISES // LINK_PROMISES, queues=QUEUES) if kind == "async" else dict(max_launches=1 << 20)
         return hz.collatz_run(kind, steps_out, starts, COUNT, **cfg)
     return run
 
@@ -309,7 +310,7 @@ def main() -> int:
     starts = hz.collatz_make_starts(SEED, BITS, COUNT)
     steps_out = hz.DeviceArray(COUNT, np.uint32)
     arena_links = ARENA_PROMISES // LINK_PROMISES
-    cfg = dict(arena=arena_links, groups=args.groups)
+    cfg = dict(arena=arena_links, groups=args.groups, queues=QUEUES)
 
     for _ in range(args.warmup):
         hz.collatz_run("async", steps_out, starts, COUNT, **cfg)

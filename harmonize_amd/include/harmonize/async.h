@@ -391,14 +391,14 @@ class AsyncProgram {
     __device__ void set_flags(unsigned int bits) { hz::wave::atomic_or(&_dev_ctx.status->flags, bits); }
 
     // Records the first bounded wait that gave up and flags the run.
-    __device__ void stall(unsigned int site, unsigned long long detail) {
+    __device__ HZ_INLINE void stall(unsigned int site, unsigned long long detail) {
         if (atomicCAS(&_dev_ctx.status->stall_site, 0u, site) == 0u) {
             hz::wave::atomic_store(&_dev_ctx.status->stall_detail, detail);
         }
         set_flags(QUEUE_STALL_FLAG);
     }
 
-    __device__ unsigned long long queue_detail(hz::LinkQueue* q, unsigned int pos) {
+    __device__ HZ_INLINE unsigned long long queue_detail(hz::LinkQueue* q, unsigned int pos) {
         unsigned long long qi = q == _dev_ctx.free_queue ? 0xFFFFFFFFull : (unsigned long long)(q - _dev_ctx.queues);
         return (qi << 32) | pos;
     }
@@ -407,7 +407,7 @@ class AsyncProgram {
 
     // ------------------------------------------------------------ link queues
     // Both functions are called by one lane.
-    __device__ bool queue_push(hz::LinkQueue* q, unsigned int* slots, unsigned int id) {
+    __device__ HZ_INLINE bool queue_push(hz::LinkQueue* q, unsigned int* slots, unsigned int id) {
         unsigned int t = hz::wave::atomic_add(&q->tail, 1u);
         unsigned int* slot = slots + (t & _dev_ctx.queue_mask);
         uint32_t spins = 0;
@@ -423,7 +423,7 @@ class AsyncProgram {
         return true;
     }
 
-    __device__ unsigned int queue_pop(hz::LinkQueue* q, unsigned int* slots) {
+    __device__ HZ_INLINE unsigned int queue_pop(hz::LinkQueue* q, unsigned int* slots) {
         if (hz::wave::atomic_load(&q->count) <= 0) return hz::LINK_NONE;
         int c = hz::wave::atomic_add_acqrel(&q->count, -1);
         if (c <= 0) {
@@ -446,13 +446,13 @@ class AsyncProgram {
     }
 
     __device__ hz::LinkQueue* work_queue(unsigned int q) { return _dev_ctx.queues + q; }
-    __device__ unsigned int* work_slots(unsigned int q) {
+    __device__ HZ_INLINE unsigned int* work_slots(unsigned int q) {
         return _dev_ctx.queue_slots + (size_t)q * (_dev_ctx.queue_mask + 1u);
     }
 
     // ------------------------------------------------------------ arena links
     // Called by one lane. Returns LINK_NONE when the arena is exhausted.
-    __device__ unsigned int arena_alloc_lane() {
+    __device__ HZ_INLINE unsigned int arena_alloc_lane() {
         GroupContext& g = _grp_ctx;
         if (g.spare_count > 0) {
             g.spare_count -= 1;
@@ -469,7 +469,7 @@ class AsyncProgram {
         return queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
     }
 
-    __device__ void arena_free_lane(unsigned int id) {
+    __device__ HZ_INLINE void arena_free_lane(unsigned int id) {
         GroupContext& g = _grp_ctx;
         if (g.spare_count < SPARES) {
             g.spare[g.spare_count] = id;
@@ -487,7 +487,7 @@ class AsyncProgram {
     // Moves stash link L to the arena and queues it. Cooperative over the
     // active lanes (all 64 in the scheduler loop, a subset inside an async
     // call). Returns false if the arena is exhausted.
-    __device__ bool spill_link(uint32_t L) {
+    __device__ HZ_INLINE bool spill_link(uint32_t L) {
         GroupContext& g = _grp_ctx;
         uint64_t act = hz::wave::active();
         uint32_t n = hz::wave::popc(act);
@@ -524,7 +524,7 @@ class AsyncProgram {
     }
 
     // Spills ready links until at least `want` stash links are free.
-    __device__ void spill_until_free(uint32_t want) {
+    __device__ HZ_INLINE void spill_until_free(uint32_t want) {
         GroupContext& g = _grp_ctx;
         while (true) {
             uint32_t fm = hz::wave::first(g.free_mask);
@@ -539,7 +539,7 @@ class AsyncProgram {
     // at least OPS+3 links and at most one open link per operation, a full
     // stash always has a ready link. Returns LINK_NONE only if the arena is
     // exhausted (the flag is then set and the caller drops its promises).
-    __device__ uint32_t take_link() {
+    __device__ HZ_INLINE uint32_t take_link() {
         GroupContext& g = _grp_ctx;
         uint32_t fm = hz::wave::first(g.free_mask);
         if (fm == 0) {
@@ -554,7 +554,7 @@ class AsyncProgram {
     }
 
     // Pulls up to `max_links` queued links into the stash with all 64 lanes.
-    __device__ uint32_t pull_links(uint32_t max_links) {
+    __device__ HZ_INLINE uint32_t pull_links(uint32_t max_links) {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         uint32_t pulled = 0;
@@ -598,7 +598,7 @@ class AsyncProgram {
     }
 
     // Appends one promise per active lane to the open link of operation D.
-    __device__ void stash_append(uint32_t D, const uint32_t* words) {
+    __device__ HZ_INLINE void stash_append(uint32_t D, const uint32_t* words) {
         GroupContext& g = _grp_ctx;
         uint64_t act = hz::wave::active();
         uint32_t n = hz::wave::popc(act);
@@ -651,7 +651,7 @@ class AsyncProgram {
     }
 
     template <typename TYPE>
-    __device__ void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
+    __device__ HZ_INLINE void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
         static_assert(Lookup<TYPE>::type::CONTAINED, "operation is not part of this program's OpSet");
         constexpr OpDisc D = Lookup<TYPE>::type::DISC;
         PromiseUnionType pu;
@@ -662,12 +662,12 @@ class AsyncProgram {
     }
 
     template <typename TYPE>
-    __device__ void immediate_call_cast(Promise<TYPE> param_value) {
+    __device__ HZ_INLINE void immediate_call_cast(Promise<TYPE> param_value) {
         param_value(*this);
     }
 
     // Stash link with the most promises among the open ones, or LINK_NONE.
-    __device__ uint32_t fullest_open(uint32_t& count) {
+    __device__ HZ_INLINE uint32_t fullest_open(uint32_t& count) {
         GroupContext& g = _grp_ctx;
         uint32_t best = hz::LINK_NONE, best_count = 0;
         for (uint32_t d = 0; d < OPS; ++d) {
@@ -685,7 +685,7 @@ class AsyncProgram {
 
     __device__ bool stash_empty() { return hz::wave::first(_grp_ctx.free_mask) == ALL_LINKS; }
 
-    __device__ void drop_holding_if_empty() {
+    __device__ HZ_INLINE void drop_holding_if_empty() {
         GroupContext& g = _grp_ctx;
         if (stash_empty() && hz::wave::first(g.holding)) {
             if (hz::wave::lane() == 0) {
@@ -696,7 +696,7 @@ class AsyncProgram {
         }
     }
 
-    __device__ void make_work_step() {
+    __device__ HZ_INLINE void make_work_step() {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         if (lane == 0) {
@@ -707,7 +707,8 @@ class AsyncProgram {
             g.made += 1;
         }
         hz::wave::sync();
-        bool mine = PROGRAM_SPEC::make_work(*this);
+        bool mine;
+        [[clang::always_inline]] mine = PROGRAM_SPEC::make_work(*this);
         bool more = hz::wave::ballot(mine) != 0;
         if (lane == 0) {
             g.can_make_work = more ? 1u : 0u;
@@ -720,7 +721,7 @@ class AsyncProgram {
         drop_holding_if_empty();
     }
 
-    __device__ void run_link(uint32_t L) {
+    __device__ HZ_INLINE void run_link(uint32_t L) {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         uint32_t op = hz::wave::first(g.link_op[L]);
@@ -753,7 +754,7 @@ class AsyncProgram {
         drop_holding_if_empty();
     }
 
-    __device__ bool queues_hint() {
+    __device__ HZ_INLINE bool queues_hint() {
         hz::LinkQueue* q = work_queue(hz::wave::first(_grp_ctx.home));
         return hz::wave::first((int32_t)hz::wave::atomic_load(&q->count)) > 0;
     }
@@ -769,7 +770,7 @@ class AsyncProgram {
     // them (full links first, then its fullest open link) to the queues.
     static constexpr uint32_t SHARE_CHECK_PERIOD = 4;
 
-    __device__ void set_hungry(bool h) {
+    __device__ HZ_INLINE void set_hungry(bool h) {
         if constexpr (!SHARE_WORK) return;
         GroupContext& g = _grp_ctx;
         if ((hz::wave::first(g.hungry) != 0) == h) return;
@@ -780,7 +781,7 @@ class AsyncProgram {
         hz::wave::sync();
     }
 
-    __device__ void share_links(uint32_t keep) {
+    __device__ HZ_INLINE void share_links(uint32_t keep) {
         if constexpr (!SHARE_WORK) return;
         GroupContext& g = _grp_ctx;
         const uint32_t others = ALL_LINKS & ~hz::wave::first(g.free_mask) & ~(1u << keep);
@@ -825,7 +826,7 @@ class AsyncProgram {
     // its first such cycle of a launch and then every QUEUE_CHECK_PERIOD-th.
     static constexpr uint32_t QUEUE_CHECK_PERIOD = 16;
 
-    __device__ CycleResult exec_cycle() {
+    __device__ HZ_INLINE CycleResult exec_cycle() {
         GroupContext& g = _grp_ctx;
         HZ_PHASE(PHASE_CYCLE);
         {
@@ -915,7 +916,7 @@ class AsyncProgram {
         return RUN;
     }
 
-    __device__ void init_group() {
+    __device__ HZ_INLINE void init_group() {
         GroupContext& g = _grp_ctx;
         if (hz::wave::lane() == 0) {
             g.free_mask = ALL_LINKS;
@@ -939,14 +940,14 @@ class AsyncProgram {
         hz::wave::sync();
     }
 
-    __device__ void init_thread() {
+    __device__ HZ_INLINE void init_thread() {
         _thd_ctx.thread_id = blockIdx.x * blockDim.x + threadIdx.x;
         _thd_ctx.rand_state = _thd_ctx.thread_id;
     }
 
     // Leaves nothing in LDS: every stash link goes back to the arena
     // (reference cleanup_runtime, program/async.h:2323-2393).
-    __device__ void cleanup_runtime() {
+    __device__ HZ_INLINE void cleanup_runtime() {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         if (lane == 0) {
@@ -1007,10 +1008,12 @@ class AsyncProgram {
     __device__ void init_program() {}
 
     // The persistent loop (reference program/async.h:2625-2668).
-    __device__ void exec(unsigned int cycle_count) {
+    __device__ HZ_INLINE void exec(unsigned int cycle_count) {
         init_group();
         init_thread();
-        PROGRAM_SPEC::initialize(*this);
+        // The spec hooks are inlined: a hook left out of line would take the
+        // program object and push the state it refers to into scratch memory.
+        [[clang::always_inline]] PROGRAM_SPEC::initialize(*this);
         for (unsigned int cycle = 0; cycle < cycle_count; ++cycle) {
             if ((cycle & 63u) == 63u) {
                 unsigned int flags = 0;
@@ -1020,27 +1023,27 @@ class AsyncProgram {
             }
             if (exec_cycle() == DONE) break;
         }
-        PROGRAM_SPEC::finalize(*this);
+        [[clang::always_inline]] PROGRAM_SPEC::finalize(*this);
         cleanup_runtime();
     }
 
     template <typename TYPE, typename... ARGS>
-    __device__ void async(ARGS... args) {
+    __device__ HZ_INLINE void async(ARGS... args) {
         async_call_cast<TYPE>(0, Promise<TYPE>(args...));
     }
 
     template <typename TYPE>
-    __device__ void async_call(Promise<TYPE> promise) {
+    __device__ HZ_INLINE void async_call(Promise<TYPE> promise) {
         async_call_cast<TYPE>(0, promise);
     }
 
     template <typename TYPE, typename... ARGS>
-    __device__ void sync(ARGS... args) {
+    __device__ HZ_INLINE void sync(ARGS... args) {
         immediate_call_cast<TYPE>(Promise<TYPE>(args...));
     }
 
     template <typename TYPE>
-    __device__ void sync_call(Promise<TYPE> promise) {
+    __device__ HZ_INLINE void sync_call(Promise<TYPE> promise) {
         immediate_call_cast<TYPE>(promise);
     }
 
@@ -1057,7 +1060,7 @@ class AsyncProgram {
     // per-lane loop the installing lane's exit branch would run only after
     // all of the wave's lanes had left the loop.)
     template <typename BARRIER, typename OP_TYPE>
-    __device__ void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
+    __device__ HZ_INLINE void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
         constexpr OpDisc D = Lookup<OP_TYPE>::type::DISC;
         bool run_now = hz::wave::atomic_load(&b.semaphore.sem) == 0u;
         bool pending = !run_now;
@@ -1146,7 +1149,7 @@ class AsyncProgram {
     }
 
     template <typename BARRIER>
-    __device__ void barrier_release(BARRIER& b) {
+    __device__ HZ_INLINE void barrier_release(BARRIER& b) {
         constexpr unsigned int NOPS = BARRIER::TYPE_COUNT;
         for (unsigned int d = 0; d < NOPS; ++d) {
             unsigned long long old =
@@ -1170,7 +1173,7 @@ class AsyncProgram {
 
    protected:
     template <unsigned int I>
-    __device__ void async_enum(PromiseEnum<OpSet>& pe) {
+    __device__ HZ_INLINE void async_enum(PromiseEnum<OpSet>& pe) {
         if constexpr (I < OPS) {
             using Op = typename hz::OpOf<I, OpSet>::type;
             if (pe.disc == I) {
@@ -1182,7 +1185,7 @@ class AsyncProgram {
     }
 
     // Any-lane arena allocation (the wave's spare cache is single-lane).
-    __device__ unsigned int arena_alloc_any() {
+    __device__ HZ_INLINE unsigned int arena_alloc_any() {
         unsigned int id = queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
         if (id != hz::LINK_NONE) return id;
         if (hz::wave::atomic_load(&_dev_ctx.status->claim) < _dev_ctx.links) {
@@ -1192,21 +1195,21 @@ class AsyncProgram {
         return queue_pop(_dev_ctx.free_queue, _dev_ctx.free_slots);
     }
 
-    __device__ void write_arena_slot(unsigned int L, unsigned int slot, const uint32_t* words) {
+    __device__ HZ_INLINE void write_arena_slot(unsigned int L, unsigned int slot, const uint32_t* words) {
         unsigned int* dst = arena_link(L);
 #pragma unroll
         for (uint32_t w = 0; w < WORDS; ++w) dst[w * 64u + slot] = words[w];
     }
 
     template <typename BARRIER>
-    __device__ void barrier_write(BARRIER& b, unsigned int L, unsigned int slot, OpDisc d, const uint32_t* words) {
+    __device__ HZ_INLINE void barrier_write(BARRIER& b, unsigned int L, unsigned int slot, OpDisc d, const uint32_t* words) {
         write_arena_slot(L, slot, words);
         barrier_finish(b, L, d);
     }
 
     // Counts one written slot; the lane that completes the link hands it on.
     template <typename BARRIER>
-    __device__ void barrier_finish(BARRIER& b, unsigned int L, OpDisc d) {
+    __device__ HZ_INLINE void barrier_finish(BARRIER& b, unsigned int L, OpDisc d) {
         hz::wave::fence_release();
         unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], 1u);
         unsigned int filled = (before & 0xFFFFu) + 1u;
@@ -1217,7 +1220,7 @@ class AsyncProgram {
     // A completed link goes onto the barrier's full list, or to the work
     // queues once the barrier is released.
     template <typename BARRIER>
-    __device__ void barrier_complete(BARRIER& b, unsigned int L, OpDisc d, unsigned int count) {
+    __device__ HZ_INLINE void barrier_complete(BARRIER& b, unsigned int L, OpDisc d, unsigned int count) {
         _dev_ctx.link_meta[L] = d | (count << 16);
         hz::wave::fence_release();
         unsigned int h = hz::wave::atomic_load(&b.full_head);
@@ -1239,14 +1242,14 @@ class AsyncProgram {
    public:
     // The async program has no queue fill fraction (reference async.h:2900-2904).
     template <typename TYPE>
-    __device__ float load_fraction() {
+    __device__ HZ_INLINE float load_fraction() {
         return NAN;
     }
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
 
     // One workgroup = GROUPS_PER_BLOCK independent groups (wavefronts).
-    __device__ static void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int cycles) {
+    __device__ static HZ_INLINE void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int cycles) {
         __shared__ GroupContext gctx[GROUPS_PER_BLOCK];
         __shared__ GroupState group_state[GROUPS_PER_BLOCK];
         const unsigned int g = (GROUPS_PER_BLOCK == 1) ? 0u : (threadIdx.x / 64u);

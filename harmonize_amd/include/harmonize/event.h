@@ -292,7 +292,9 @@ class EventProgram {
         const uint32_t home = home_segment();
         unsigned long long made = 0;
 
-        PROGRAM_SPEC::initialize(*this);
+        // The spec hooks are inlined: a hook left out of line would take the
+        // program object and push the state it refers to into scratch memory.
+        [[clang::always_inline]] PROGRAM_SPEC::initialize(*this);
 
         while (true) {
             // Claim a chunk: home segment first, then the other non-empty ones.
@@ -328,7 +330,8 @@ class EventProgram {
                         if (n >= _dev_ctx.seg_capacity / 2) more = false;
                     }
                     if (!more) break;
-                    bool mine = PROGRAM_SPEC::make_work(*this);
+                    bool mine;
+                    [[clang::always_inline]] mine = PROGRAM_SPEC::make_work(*this);
                     more = hz::wave::ballot(mine) != 0;
                     if (more) made++;
                 }
@@ -342,7 +345,7 @@ class EventProgram {
             }
         }
 
-        PROGRAM_SPEC::finalize(*this);
+        [[clang::always_inline]] PROGRAM_SPEC::finalize(*this);
         if (lane == 0 && made) hz::wave::atomic_add(&st->make_work_calls, made);
     }
 

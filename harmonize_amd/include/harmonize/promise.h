@@ -218,7 +218,7 @@ struct Unpack;
 template <typename OP, typename PROGRAM, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<>, UNROLLED...> {
     template <typename R>
-    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<>&, UNROLLED... u) {
+    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<>&, UNROLLED... u) {
         return OP::template eval<PROGRAM>(prog, u...);
     }
 };
@@ -226,7 +226,7 @@ struct Unpack<OP, PROGRAM, ArgTuple<>, UNROLLED...> {
 template <typename OP, typename PROGRAM, typename H, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<H>, UNROLLED...> {
     template <typename R>
-    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<H>& t, UNROLLED... u) {
+    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<H>& t, UNROLLED... u) {
         return OP::template eval<PROGRAM>(prog, u..., t.head);
     }
 };
@@ -234,7 +234,7 @@ struct Unpack<OP, PROGRAM, ArgTuple<H>, UNROLLED...> {
 template <typename OP, typename PROGRAM, typename H, typename N, typename... T, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<H, N, T...>, UNROLLED...> {
     template <typename R>
-    __device__ __host__ static R call(PROGRAM prog, const ArgTuple<H, N, T...>& t, UNROLLED... u) {
+    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<H, N, T...>& t, UNROLLED... u) {
         return Unpack<OP, PROGRAM, ArgTuple<N, T...>, UNROLLED..., H>::template call<R>(prog, t.tail, u..., t.head);
     }
 };
@@ -256,7 +256,7 @@ struct Promise : hz::detail::PromiseReturn<typename OpType<typename OPERATION::T
 
     // Evaluates the operation with the stored arguments on the given program.
     template <typename PROGRAM>
-    __device__ __host__ Return operator()(PROGRAM program) {
+    __device__ __host__ HZ_INLINE Return operator()(PROGRAM program) {
         return hz::detail::Unpack<OPERATION, PROGRAM, Args>::template call<Return>(program, args);
     }
 };
@@ -311,7 +311,7 @@ union PromiseUnion<OpUnion<HEAD, TAIL...>> {
 
     // Views the union as the promise type of TYPE.
     template <typename TYPE>
-    HZ_HD Promise<TYPE>& cast() {
+    HZ_HD HZ_INLINE Promise<TYPE>& cast() {
         static_assert(OpUnionLookup<TYPE, OpSet>::CONTAINED,
                       "Promise type is not part of this promise union's operation set.");
         if constexpr (std::is_same<TYPE, HEAD>::value) {
@@ -323,13 +323,13 @@ union PromiseUnion<OpUnion<HEAD, TAIL...>> {
 
     // Evaluates the union as a promise of the statically known operation TYPE.
     template <typename PROGRAM, typename TYPE>
-    __device__ __host__ void rigid_eval(PROGRAM program) {
+    __device__ __host__ HZ_INLINE void rigid_eval(PROGRAM program) {
         cast<TYPE>()(program);
     }
 
     // Evaluates the union as a promise of the operation with discriminant `disc`.
     template <typename PROGRAM>
-    __device__ __host__ void loose_eval(PROGRAM program, OpDisc disc) {
+    __device__ __host__ HZ_INLINE void loose_eval(PROGRAM program, OpDisc disc) {
         if (disc == 0) {
             head_form(program);
         } else {
