@@ -234,6 +234,7 @@ class AsyncProgram {
         uint32_t queue_check;  // cycles until the device queues are consulted again
         uint32_t share_check;  // cycles until the hungry count is consulted again
         uint32_t hungry;       // this group is counted in status->hungry
+        uint32_t no_makers;    // this group has seen (and acquired) makers == 0
 #ifdef HZ_PHASE_TIMING
         unsigned long long int phase[8];
 #endif
@@ -403,29 +404,45 @@ class AsyncProgram {
         return (qi << 32) | pos;
     }
 
-    __device__ void add_units(int delta) { hz::wave::atomic_add_acqrel(&_dev_ctx.status->units, delta); }
+    // Memory ordering. On gfx950 an agent-scope release is an L2 write-back
+    // (buffer_wbl2) and an acquire an invalidation (buffer_inv) that other
+    // wavefronts on the XCD then pay for in cache misses, so the runtime
+    // orders only what carries data: a link is published by one release
+    // after its words and metadata are written, and consumed by one acquire
+    // before they are read. Counters (units, queue counts, heads, tails) are
+    // relaxed; the termination check synchronises with the makers count once
+    // it reads zero (see exec_cycle).
+    __device__ void add_units(int delta) { hz::wave::atomic_add(&_dev_ctx.status->units, delta); }
 
     // ------------------------------------------------------------ link queues
-    // Both functions are called by one lane.
-    __device__ HZ_INLINE bool queue_push(hz::LinkQueue* q, unsigned int* slots, unsigned int id) {
+    // Both functions are called by one lane. `publish`: store the id with
+    // release semantics (the caller has not fenced the link's data itself).
+    __device__ HZ_INLINE bool queue_push(hz::LinkQueue* q, unsigned int* slots, unsigned int id,
+                                         bool publish = true) {
         unsigned int t = hz::wave::atomic_add(&q->tail, 1u);
         unsigned int* slot = slots + (t & _dev_ctx.queue_mask);
         uint32_t spins = 0;
-        while (hz::wave::atomic_load_acquire(slot) != hz::LINK_NONE) {
+        while (hz::wave::atomic_load(slot) != hz::LINK_NONE) {
             if (++spins > SLOT_SPIN_LIMIT) {
                 stall(hz::STALL_QUEUE_PUSH, queue_detail(q, t));
                 return false;
             }
             hz::wave::sleep_short();
         }
-        hz::wave::atomic_store_release(slot, id);
-        hz::wave::atomic_add_acqrel(&q->count, 1);
+        if (publish) {
+            hz::wave::atomic_store_release(slot, id);
+        } else {
+            hz::wave::atomic_store(slot, id);
+        }
+        hz::wave::atomic_add(&q->count, 1);
         return true;
     }
 
-    __device__ HZ_INLINE unsigned int queue_pop(hz::LinkQueue* q, unsigned int* slots) {
+    // `consume`: acquire the link's data (the caller reads it afterwards and
+    // does not fence itself).
+    __device__ HZ_INLINE unsigned int queue_pop(hz::LinkQueue* q, unsigned int* slots, bool consume = false) {
         if (hz::wave::atomic_load(&q->count) <= 0) return hz::LINK_NONE;
-        int c = hz::wave::atomic_add_acqrel(&q->count, -1);
+        int c = hz::wave::atomic_add(&q->count, -1);
         if (c <= 0) {
             hz::wave::atomic_add(&q->count, 1);
             return hz::LINK_NONE;
@@ -434,14 +451,15 @@ class AsyncProgram {
         unsigned int* slot = slots + (h & _dev_ctx.queue_mask);
         unsigned int id;
         uint32_t spins = 0;
-        while ((id = hz::wave::atomic_load_acquire(slot)) == hz::LINK_NONE) {
+        while ((id = hz::wave::atomic_load(slot)) == hz::LINK_NONE) {
             if (++spins > SLOT_SPIN_LIMIT) {
                 stall(hz::STALL_QUEUE_POP, queue_detail(q, h));
                 return hz::LINK_NONE;
             }
             hz::wave::sleep_short();
         }
-        hz::wave::atomic_store_release(slot, hz::LINK_NONE);
+        if (consume) hz::wave::fence_acquire();
+        hz::wave::atomic_store(slot, hz::LINK_NONE);
         return id;
     }
 
@@ -475,7 +493,7 @@ class AsyncProgram {
             g.spare[g.spare_count] = id;
             g.spare_count += 1;
         } else {
-            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, id);
+            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, id, false);
         }
     }
 
@@ -509,12 +527,13 @@ class AsyncProgram {
         }
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
+        if (r == 0) _dev_ctx.link_meta[id] = op | (cnt << 16);
+        // One release for the link's words and metadata; the id is then
+        // stored without a second one.
         hz::wave::fence_release();
         if (r == 0) {
-            _dev_ctx.link_meta[id] = op | (cnt << 16);
-            hz::wave::fence_release();
             add_units(1);
-            queue_push(work_queue(g.home), work_slots(g.home), id);
+            queue_push(work_queue(g.home), work_slots(g.home), id, false);
             g.free_mask |= (1u << L);
             g.ready_mask &= ~(1u << L);
             g.spills += 1;
@@ -894,11 +913,22 @@ class AsyncProgram {
         set_hungry(true);
         uint32_t streak = hz::wave::first(g.idle_streak);
         if (streak >= IDLE_EXIT_POLLS) return DONE;
+        // makers only falls to zero after every maker's last units increment
+        // (its decrement is a release), so once a relaxed read of makers sees
+        // zero, one acquire makes those increments visible; from then on
+        // units is read relaxed: with no makers it can only fall, and it
+        // stays at zero once there.
         int makers = 0, units = 0;
         unsigned int flags = 0;
         if (hz::wave::lane() == 0) {
-            makers = hz::wave::atomic_load_acquire(&_dev_ctx.status->makers);
-            units = hz::wave::atomic_load_acquire(&_dev_ctx.status->units);
+            if (!g.no_makers) {
+                makers = hz::wave::atomic_load(&_dev_ctx.status->makers);
+                if (makers == 0) {
+                    hz::wave::fence_acquire();
+                    g.no_makers = 1;
+                }
+            }
+            units = hz::wave::atomic_load(&_dev_ctx.status->units);
             flags = hz::wave::atomic_load(&_dev_ctx.status->flags);
             g.idle += 1;
             g.idle_streak = streak + 1;
@@ -932,6 +962,7 @@ class AsyncProgram {
             g.queue_check = 0;
             g.share_check = 0;
             g.hungry = 0;
+            g.no_makers = 0;
 #ifdef HZ_PHASE_TIMING
             for (int i = 0; i < 8; ++i) g.phase[i] = 0;
 #endif
@@ -975,7 +1006,7 @@ class AsyncProgram {
             }
             while (g.spare_count > 0) {
                 g.spare_count -= 1;
-                queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, g.spare[g.spare_count]);
+                queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, g.spare[g.spare_count], false);
             }
             if (g.maker) {
                 if (g.can_make_work) set_flags(EARLY_HALT_FLAG);
@@ -1134,7 +1165,7 @@ class AsyncProgram {
                         if (installed) {
                             barrier_finish(b, fresh, D);
                         } else if (released) {  // released meanwhile: run it now
-                            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh);
+                            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh, false);
                             run_now = true;
                         } else {
                             stall(hz::STALL_BARRIER_INSTALL, cur);
