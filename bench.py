@@ -119,8 +119,9 @@ BFS_GRAPHS = 64
 
 def bfs_comparison(hz, repeats: int = 3) -> dict:
     """64 R-MAT graphs (2^14..2^22 vertices, 16 edges per vertex) searched by
-    one program: AsyncProgram through its captured hipGraph vs EventProgram,
-    each the best of `repeats` runs. Levels must agree between the two."""
+    one program: AsyncProgram through its captured hipGraph, at the
+    configuration the autotuner picks for this batch, vs EventProgram, each the
+    best of `repeats` runs. Levels must agree between the two."""
     sizes = hz.bfs_batch_sizes(BFS_SEED, BFS_GRAPHS, 14.0, 22.0)
     batch = hz.BfsBatch.rmat(BFS_SEED, sizes)
     try:
@@ -130,7 +131,10 @@ def bfs_comparison(hz, repeats: int = 3) -> dict:
                 runs.append(batch.run(kind, max_launches=1 << 16, **cfg))
             return min(runs, key=lambda r: r["elapsed_ms"]), batch.levels()
 
-        a, lev_a = best("async", flags=hz.RUN_GRAPH)
+        tuned = hz.tune_bfs(batch, rounds=1, reps=1)
+        cfg = {k: v for k, v in tuned["best"].items() if v and k != "max_launches"}
+        cfg["flags"] = hz.RUN_GRAPH
+        a, lev_a = best("async", **cfg)
         e, lev_e = best("event")
         if not np.array_equal(lev_a, lev_e):
             raise RuntimeError("bfs: async and event levels differ")
@@ -147,7 +151,8 @@ def bfs_comparison(hz, repeats: int = 3) -> dict:
             "build_ms": batch.build_ms,
             "async": {"visits": a["promises"], "ms": a["elapsed_ms"], "launches": a["launches"],
                       "groups": a["groups"], "promises_per_s": rate(a), "teps": traversed / (a["elapsed_ms"] * 1e-3),
-                      "launch": "hipGraph"},
+                      "launch": "hipGraph", "tuned_config": cfg, "untuned_ms": tuned["start_ms"],
+                      "tuning_trials": tuned["trials"]},
             "event": {"visits": e["promises"], "ms": e["elapsed_ms"], "launches": e["launches"],
                       "groups": e["groups"], "promises_per_s": rate(e), "teps": traversed / (e["elapsed_ms"] * 1e-3)},
             "async_vs_event_time": e["elapsed_ms"] / a["elapsed_ms"],

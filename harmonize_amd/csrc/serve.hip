@@ -364,7 +364,10 @@ struct hz_server {
                 st.split = BFS_DEFAULT_SPLIT;
                 s.bfs->device_state = st;
                 s.bfs->reset(s.stream);
-                j.groups = job_groups(groups_bfs, V);
+                // Generations are ordinary (non-persistent) launches: a job may
+                // ask for the whole device and the hardware shares it between
+                // the slots' streams.
+                j.groups = job_groups(groups_bfs * cfg.slots, V);
                 enqueue_generations(s, j.groups);
                 break;
             }
