@@ -173,7 +173,7 @@ def traversed_edges(hz, batch, reached) -> int:
     return total
 
 
-SERVE_JOBS = 300
+SERVE_JOBS = 400
 
 
 def serving_mix(n: int, seed: int) -> list:
@@ -189,53 +189,55 @@ def serving_mix(n: int, seed: int) -> list:
 
 def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -> dict:
     """Peak throughput with every job arriving at once, then open-loop Poisson
-    arrivals at fractions of that peak: p50/p99 latency (arrival to completion
-    seen by the server) and achieved throughput at each load."""
-    srv = hz.Server(pool_bytes=48 << 30, slots=4)
-    try:
-        warm = serving_mix(12, 99)
-        srv.run_trace(warm, np.zeros(len(warm)))
-        mix = serving_mix(jobs, 1)
-        res = srv.run_trace(mix, np.zeros(len(mix)))
-        span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
-        peak = len(res) / (span * 1e-3)
-        promises = sum(r["promises"] for r in res)
-        out = {
-            "workload": f"{jobs} jobs, Collatz / photon walk / BFS with p = 1/3 each, sizes log-uniform in "
-                        f"[2^10, 2^22] items, per-job seeds; 4 slots (concurrent streams)",
-            "peak_jobs_per_s": peak,
-            "peak_promises_per_s": promises / (span * 1e-3),
-            "peak_makespan_ms": span,
-            "loads": [],
-        }
-        rng = np.random.default_rng(2)
-        for k, load in enumerate(loads):
-            mix = serving_mix(jobs, 10 + k)
-            rate = load * peak  # jobs per second
-            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(mix))) * 1e3
-            arrivals -= arrivals[0]
-            res = srv.run_trace(mix, arrivals)
-            lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+    arrivals of the same job mix at fractions of that peak: p50/p99 latency
+    (arrival to completion seen by the server) and achieved throughput at each
+    load, for arrival-order and largest-first admission."""
+    out = {
+        "workload": f"{jobs} jobs, Collatz / photon walk / BFS with p = 1/3 each, sizes log-uniform in "
+                    f"[2^10, 2^22] items, per-job seeds; 4 slots (concurrent streams); the same mix at every load",
+    }
+    mix = serving_mix(jobs, 1)
+    unit = np.random.default_rng(2).exponential(1.0, size=len(mix))  # Poisson gaps at rate 1
+    for name, policy in (("fifo", hz.SERVE_FIFO), ("largest_first", hz.SERVE_LARGEST_FIRST)):
+        srv = hz.Server(pool_bytes=48 << 30, slots=4, policy=policy)
+        try:
+            warm = serving_mix(12, 99)
+            srv.run_trace(warm, np.zeros(len(warm)))
+            res = srv.run_trace(mix, np.zeros(len(mix)))
             span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
-            by_kind = {}
-            for kind in ("collatz", "photon", "bfs"):
-                sel = np.array([m[0] == kind for m in mix])
-                if sel.any():
+            peak = len(res) / (span * 1e-3)
+            entry = {
+                "peak_jobs_per_s": peak,
+                "peak_promises_per_s": sum(r["promises"] for r in res) / (span * 1e-3),
+                "peak_makespan_ms": span,
+                "loads": [],
+            }
+            for load in loads:
+                rate = load * peak  # jobs per second
+                arrivals = np.cumsum(unit / rate) * 1e3
+                arrivals -= arrivals[0]
+                res = srv.run_trace(mix, arrivals)
+                lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+                span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
+                by_kind = {}
+                for kind in ("collatz", "photon", "bfs"):
+                    sel = np.array([m[0] == kind for m in mix])
                     by_kind[kind] = {"p50_ms": float(np.percentile(lat[sel], 50)),
                                      "p99_ms": float(np.percentile(lat[sel], 99))}
-            out["loads"].append({
-                "offered_fraction_of_peak": load,
-                "offered_jobs_per_s": rate,
-                "achieved_jobs_per_s": len(res) / (span * 1e-3),
-                "achieved_promises_per_s": sum(r["promises"] for r in res) / (span * 1e-3),
-                "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)),
-                "max_ms": float(lat.max()),
-                "by_kind": by_kind,
-            })
-        return out
-    finally:
-        srv.close()
+                entry["loads"].append({
+                    "offered_fraction_of_peak": load,
+                    "offered_jobs_per_s": rate,
+                    "achieved_jobs_per_s": len(res) / (span * 1e-3),
+                    "achieved_promises_per_s": sum(r["promises"] for r in res) / (span * 1e-3),
+                    "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)),
+                    "max_ms": float(lat.max()),
+                    "by_kind": by_kind,
+                })
+            out[name] = entry
+        finally:
+            srv.close()
+    return out
 
 
 def _leg_collatz(kind):

@@ -532,8 +532,11 @@ class ServerConfig(ctypes.Structure):
         ("slots", ctypes.c_uint32),
         ("max_pending", ctypes.c_uint32),
         ("keep_outputs", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("policy", ctypes.c_uint32),
     ]
+
+
+SERVE_FIFO, SERVE_LARGEST_FIRST = 0, 1
 
 
 class Job(ctypes.Structure):
@@ -620,8 +623,8 @@ class Server:
     }
 
     def __init__(self, pool_bytes: int = 0, slots: int = 0, arena_links: int = 0, keep_outputs: bool = False,
-                 max_pending: int = 0):
-        cfg = ServerConfig(pool_bytes, arena_links, slots, max_pending, 1 if keep_outputs else 0, 0)
+                 max_pending: int = 0, policy: int = SERVE_FIFO):
+        cfg = ServerConfig(pool_bytes, arena_links, slots, max_pending, 1 if keep_outputs else 0, policy)
         h = _VP()
         check(lib().hz_server_create(ctypes.byref(cfg), ctypes.byref(h)), "hz_server_create")
         self.handle = h.value

@@ -419,6 +419,20 @@ struct hz_server {
         }
     }
 
+    // The pending unit to admit next: the oldest (HZ_SERVE_FIFO), or the one
+    // of the largest job, oldest first among equals (HZ_SERVE_LARGEST_FIRST);
+    // a job's chunks stay in order either way.
+    size_t next_unit() const {
+        if (cfg.policy != HZ_SERVE_LARGEST_FIRST) return 0;
+        size_t best = 0;
+        for (size_t i = 1; i < pending.size(); ++i) {
+            const Job& a = jobs[(size_t)pending[i].first];
+            const Job& b = jobs[(size_t)pending[best].first];
+            if (a.desc.size > b.desc.size) best = i;
+        }
+        return best;
+    }
+
     // Reaps finished passes and admits pending chunks into free slots.
     void progress() {
         for (auto& sp : slots) {
@@ -454,21 +468,22 @@ struct hz_server {
             Slot& s = *sp;
             if (s.job >= 0) continue;
             while (!pending.empty()) {
-                auto [id, c] = pending.front();
+                size_t pick = next_unit();
+                auto [id, c] = pending[pick];
                 Job& j = jobs[(size_t)id];
                 if (j.status == HZ_JOB_FAILED) {  // drop the rest of a failed job
-                    pending.pop_front();
+                    pending.erase(pending.begin() + (std::ptrdiff_t)pick);
                     continue;
                 }
                 if (j.off == Pool::NONE) {
                     uint64_t off = pool->alloc(j.layout.bytes);
-                    if (off == Pool::NONE) return;  // the head waits for memory; arrival order is kept
+                    if (off == Pool::NONE) return;  // the chosen unit waits for memory; nothing overtakes it
                     j.off = off;
                     j.status = HZ_JOB_RUNNING;
                     j.start_ms = now_ms(t0);
                     j.slot = (uint32_t)(&sp - &slots[0]);
                 }
-                pending.pop_front();
+                pending.erase(pending.begin() + (std::ptrdiff_t)pick);
                 s.job = (int64_t)j.id;
                 s.chunk = c;
                 j.chunks_started++;

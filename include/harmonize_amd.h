@@ -333,8 +333,10 @@ int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tu
  * so admission allocates nothing. A Collatz or photon job of more than 2^18
  * items is cut into up to `slots` chunks of contiguous items; a BFS job runs
  * as one unit under the EventProgram. Job state lives in a device pool; a
- * job's state is allocated when its first unit is admitted, in arrival
- * order, once a slot is free and the state fits. Jobs:
+ * job's state is allocated when its first unit is admitted, once a slot is
+ * free and the state fits. Units are admitted in arrival order, or largest
+ * job first (config.policy); the unit chosen waits for memory rather than
+ * being overtaken. Jobs:
  *   HZ_JOB_COLLATZ  size starts from Philox4x32-10 keyed by seed, uniform in
  *                   [1, 2^32) (hz_collatz_make_starts with bits = 32);
  *                   output 0: u32 steps, output 1: u64 starts.
@@ -361,8 +363,11 @@ typedef struct hz_server_config {
     uint32_t slots;       /* concurrent jobs, 1..8 (0 = 4, the hardware queues of a process) */
     uint32_t max_pending; /* submissions beyond this many waiting jobs fail (0 = no bound) */
     uint32_t keep_outputs; /* 1: a finished job keeps its state until hz_server_release */
-    uint32_t reserved;
+    uint32_t policy;       /* admission order: HZ_SERVE_FIFO (0) or HZ_SERVE_LARGEST_FIRST */
 } hz_server_config;
+
+#define HZ_SERVE_FIFO 0u
+#define HZ_SERVE_LARGEST_FIRST 1u
 typedef struct hz_job {
     uint32_t type; /* HZ_JOB_* */
     uint32_t size; /* work items: starts, photons or vertices */

@@ -143,3 +143,23 @@ def test_rejects_bad_jobs(hz):
         assert srv.job(i)["status"] == hz.JOB_DONE
     finally:
         srv.close()
+
+
+def test_largest_first_policy(hz):
+    # One slot, three jobs queued together: largest-first admits the largest
+    # job first; arrival order admits them as they came.
+    jobs = [("collatz", 2000, 1), ("collatz", 200000, 2), ("photon", 30000, 3)]
+    order = {}
+    for policy in (hz.SERVE_FIFO, hz.SERVE_LARGEST_FIRST):
+        srv = hz.Server(pool_bytes=1 << 30, slots=1, policy=policy, keep_outputs=True)
+        try:
+            blocker = srv.submit("collatz", 50000, 9)  # occupies the slot while the others queue
+            res = srv.run_trace(jobs, [0.0, 0.0, 0.0])
+            order[policy] = [j for _, j in sorted((r["start_ms"], i) for i, r in enumerate(res))]
+            want, _ = O.collatz_steps(O.collatz_make_starts(2, 32, 200000))
+            assert np.array_equal(srv.output(res[1]["id"], 0), want)
+            assert srv.job(blocker)["status"] == hz.JOB_DONE
+        finally:
+            srv.close()
+    assert order[hz.SERVE_FIFO] == [0, 1, 2]
+    assert order[hz.SERVE_LARGEST_FIRST] == [1, 2, 0]
