@@ -689,3 +689,79 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ generic instances
+
+SPEC_COLLATZ, SPEC_PHOTON, SPEC_BFS, SPEC_BARRIER = 0, 1, 2, 3
+
+_SIGNATURES.update(
+    {
+        "hz_instance_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _VP, ctypes.c_size_t,
+                                              ctypes.POINTER(_VP)]),
+        "hz_instance_destroy": (ctypes.c_int, [_VP]),
+        "hz_init": (ctypes.c_int, [_VP, ctypes.c_uint32]),
+        "hz_exec": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32]),
+        "hz_complete": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+        "hz_clear_flags": (ctypes.c_int, [_VP]),
+        "hz_state_load": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),
+        "hz_state_store": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),
+        "hz_set_device": (ctypes.c_int, [ctypes.c_int]),
+        "hz_serve_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),
+        "hz_serve_poll": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    }
+)
+
+
+class CollatzState(ctypes.Structure):
+    _fields_ = [("starts", _VP), ("steps", _VP), ("iterator", _VP)]
+
+
+class Instance:
+    """A program instance chosen at run time: (kind, spec) with a state struct."""
+
+    def __init__(self, kind, spec: int, size: int, state: ctypes.Structure):
+        k = KINDS[kind] if isinstance(kind, str) else int(kind)
+        h = _VP()
+        check(lib().hz_instance_create(k, spec, size, ctypes.byref(state), ctypes.sizeof(state), ctypes.byref(h)),
+              "hz_instance_create")
+        self.handle = h.value
+        self.state_type = type(state)
+
+    def init(self, groups: int) -> None:
+        check(lib().hz_init(self.handle, groups), "hz_init")
+
+    def exec(self, groups: int, cycles: int) -> None:
+        check(lib().hz_exec(self.handle, groups, cycles), "hz_exec")
+
+    def complete(self) -> bool:
+        d = ctypes.c_int()
+        check(lib().hz_complete(self.handle, ctypes.byref(d)), "hz_complete")
+        return d.value == 1
+
+    def load_state(self):
+        st = self.state_type()
+        check(lib().hz_state_load(self.handle, 0, ctypes.sizeof(st), ctypes.byref(st)), "hz_state_load")
+        return st
+
+    def store_state(self, st) -> None:
+        check(lib().hz_state_store(self.handle, 0, ctypes.sizeof(st), ctypes.byref(st)), "hz_state_store")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().hz_instance_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_poll(server: "Server", capacity: int = 64) -> list:
+    """Ids of the server's jobs that finished since the previous poll."""
+    ids = np.zeros(capacity, dtype=np.uint64)
+    n = ctypes.c_uint32()
+    check(lib().hz_serve_poll(server.handle, ids.ctypes.data, capacity, ctypes.byref(n)), "hz_serve_poll")
+    return [int(x) for x in ids[: n.value]]

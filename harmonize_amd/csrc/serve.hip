@@ -232,6 +232,7 @@ struct hz_server {
     std::unique_ptr<Pool> pool;
     std::deque<Job> jobs;  // by id
     std::deque<std::pair<uint64_t, uint32_t>> pending;  // (job, chunk) in arrival order
+    std::deque<uint64_t> finished;                      // jobs finished since the last poll
     uint32_t groups_collatz = 0, groups_photon = 0, groups_bfs = 0;
 
     struct Slot {
@@ -462,6 +463,7 @@ struct hz_server {
                     j.end_ms = now_ms(t0);
                 }
                 if (!cfg.keep_outputs || j.status == HZ_JOB_FAILED) release_memory(j);
+                finished.push_back(j.id);
             }
         }
         for (auto& sp : slots) {
@@ -494,7 +496,10 @@ struct hz_server {
                     j.end_ms = now_ms(t0);
                     s.job = -1;
                     j.chunks_done++;
-                    if (j.chunks_done == j.chunks_started) release_memory(j);
+                    if (j.chunks_done == j.chunks_started) {
+                        release_memory(j);
+                        finished.push_back(j.id);
+                    }
                     throw;
                 }
                 break;
@@ -753,5 +758,24 @@ extern "C" int hz_pool_test(uint64_t size, const int64_t* ops, uint32_t count, u
                 out[i] = p.largest_free();
             }
         }
+    });
+}
+
+// The serving entry points under the names of a request/poll interface.
+extern "C" int hz_serve_submit(hz_server* server, const hz_job* job, uint64_t* id) {
+    return hz_server_submit(server, job, id);
+}
+
+extern "C" int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        check_server(server);
+        if (!count || (capacity && !finished_ids)) throw std::invalid_argument("hz_serve_poll: null output");
+        server->progress();
+        uint32_t n = 0;
+        while (n < capacity && !server->finished.empty()) {
+            finished_ids[n++] = server->finished.front();
+            server->finished.pop_front();
+        }
+        *count = n;
     });
 }

@@ -177,6 +177,34 @@ HZ_DECLARE_PROGRAM_ABI(bfs_event)
 HZ_DECLARE_PROGRAM_ABI(bfs_async)
 HZ_DECLARE_PROGRAM_ABI(barrier_async)
 
+/* -------------------------------------------- generic instances (layer 1)
+ * One handle for any built-in specification and kind, chosen at run time
+ * (the boundary the per-program names above expand to):
+ *   spec = HZ_SPEC_COLLATZ, HZ_SPEC_PHOTON, HZ_SPEC_BFS or HZ_SPEC_BARRIER
+ *   (the barrier program exists only as HZ_ASYNC); kind = HZ_EVENT or HZ_ASYNC;
+ *   size = io capacity per operation (event) or arena links (async);
+ *   device_state = host copy of the spec's state struct (hz_collatz_state,
+ *   hz_photon_state, hz_bfs_state, hz_barrier_state), state_bytes its size.
+ * hz_exec launches once and synchronizes (the reference's convention);
+ * hz_complete sets *done to 1 when the program has finished. The state lives
+ * on the device; hz_state_load/store copy a byte range of it and take effect
+ * from the next launch. */
+#define HZ_SPEC_COLLATZ 0
+#define HZ_SPEC_PHOTON 1
+#define HZ_SPEC_BFS 2
+#define HZ_SPEC_BARRIER 3
+typedef struct hz_instance hz_instance;
+int hz_instance_create(int kind, int spec, uint64_t size, const void* device_state, size_t state_bytes,
+                       hz_instance** out);
+int hz_instance_destroy(hz_instance* instance);
+int hz_init(hz_instance* instance, uint32_t groups);
+int hz_exec(hz_instance* instance, uint32_t groups, uint32_t cycles);
+int hz_complete(hz_instance* instance, int* done);
+int hz_clear_flags(hz_instance* instance);
+int hz_state_load(hz_instance* instance, uint64_t offset, uint64_t bytes, void* host);
+int hz_state_store(hz_instance* instance, uint64_t offset, uint64_t bytes, const void* host);
+int hz_set_device(int device);
+
 /* ------------------------------------------------------ drivers (layer 2) */
 
 typedef struct hz_run_config {
@@ -421,6 +449,11 @@ int hz_server_info(const hz_server* server, hz_server_info_t* info);
  * starts; returns when all are finished. results[i] describes job i. */
 int hz_server_run_trace(hz_server* server, const hz_job* jobs, const double* arrival_ms, uint32_t count,
                         double timeout_ms, hz_job_result* results);
+/* Request/poll names: hz_serve_submit is hz_server_submit; hz_serve_poll
+ * makes progress and returns (up to `capacity`) the ids of jobs that finished
+ * (done or failed) since the previous poll. */
+int hz_serve_submit(hz_server* server, const hz_job* job, uint64_t* id);
+int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, uint32_t* count);
 /* Host-only check of the job-state pool allocator: ops[i] >= 0 allocates
  * that many bytes (out[i] = offset, or all-ones if it does not fit);
  * ops[i] = -k - 1 frees the k-th allocation (out[i] = largest free block). */
