@@ -288,6 +288,11 @@ _SIGNATURES.update(
             [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.POINTER(RunConfig),
              ctypes.POINTER(RunStats)],
         ),
+        "hz_futures_run": (
+            ctypes.c_int,
+            [ctypes.c_int, _VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.POINTER(RunConfig),
+             ctypes.POINTER(RunStats)],
+        ),
     }
 )
 
@@ -301,6 +306,26 @@ def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
     check(lib().hz_barrier_run(dep_count, non_dep_count, barrier_count, counter.ptr, hits.ptr, violations.ptr,
                                ctypes.byref(config), ctypes.byref(st)), "hz_barrier_run")
     return int(counter.numpy()[0]), hits.numpy()[:dep_count], int(violations.numpy()[0]), st.as_dict()
+
+
+FUTURES_UNRESOLVED = (1 << 64) - 1  # a reader that ran early, or a future left unresolved
+
+
+def futures_run(kind, starts: DeviceArray, readers: int = 0, **cfg):
+    """Runs the futures program (value-returning operations resolving a tree of
+    futures over the Collatz step counts of `starts`, a power-of-two count).
+    Returns (total, echo[readers], forwarded[readers], stats): total is the sum
+    of the step counts; reader r writes echo[r] = total + r after awaiting the
+    total and receives forwarded[r] = total through a forwarded future."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    echo = DeviceArray(max(1, readers), np.uint64)
+    forwarded = DeviceArray(max(1, readers), np.uint64)
+    total = ctypes.c_uint64(0)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(lib().hz_futures_run(k, starts.ptr, starts.count, readers, echo.ptr, forwarded.ptr, ctypes.byref(total),
+                               ctypes.byref(config), ctypes.byref(st)), "hz_futures_run")
+    return int(total.value), echo.numpy()[:readers], forwarded.numpy()[:readers], st.as_dict()
 
 
 def photon_params(**overrides) -> PhotonParams:

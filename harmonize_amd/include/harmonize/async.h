@@ -120,7 +120,7 @@ class AsyncProgram {
     typedef AsyncProgram<PROGRAM_SPEC> ProgramType;
 
     using AdrType = typename hz::spec::AdrTypeOr<PROGRAM_SPEC, unsigned int>::type;
-    using OpSet = typename hz::spec::OpSetOr<PROGRAM_SPEC, OpUnion<>>::type;
+    using OpSet = hz::spec::ProgramOpSet<PROGRAM_SPEC>;
     using DeviceState = typename hz::spec::DeviceStateOr<PROGRAM_SPEC, VoidState>::type;
     using GroupState = typename hz::spec::GroupStateOr<PROGRAM_SPEC, VoidState>::type;
     using ThreadState = typename hz::spec::ThreadStateOr<PROGRAM_SPEC, VoidState>::type;
@@ -186,6 +186,7 @@ class AsyncProgram {
     static const unsigned int STASH_FAIL_FLAG = 0x00000002;
     static const unsigned int ARENA_EXHAUSTED_FLAG = 0x00000008;
     static const unsigned int QUEUE_STALL_FLAG = 0x00000010;
+    static const unsigned int USAGE_FLAG = 0x00000020;
     static const unsigned int EARLY_HALT_FLAG = 0x40000000;
     static const unsigned int COMPLETION_FLAG = 0x80000000;
 
@@ -357,6 +358,11 @@ class AsyncProgram {
                 throw std::runtime_error(
                     "AsyncProgram: the link arena was exhausted and promises were lost; "
                     "construct the Instance with a larger arena");
+            }
+            if (host_status.flags & USAGE_FLAG) {
+                throw std::runtime_error(
+                    "AsyncProgram: a Return forwarded a future that cannot hold promises "
+                    "(a unit future that was not ready)");
             }
             if (host_status.flags & QUEUE_STALL_FLAG) {
                 char detail[64];
@@ -1079,7 +1085,7 @@ class AsyncProgram {
     }
 
     // Queues a promise held as a PromiseEnum (UnitBarrier release).
-    __device__ void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
+    __device__ HZ_INLINE void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
 
     // ------------------------------------------------------ remapping barrier
     // (protocol described in barrier.h). link_aux[2L] = next link in a
@@ -1176,7 +1182,12 @@ class AsyncProgram {
             }
             if (hz::wave::ballot(pending) != 0ull) hz::wave::sleep_short();
         }
-        if (run_now) async_call_cast<OP_TYPE>(0, promise);
+        if (run_now) {
+            // Pairs with the release of the semaphore update that released
+            // the barrier, so the promise sees what was written before it.
+            hz::wave::fence_acquire();
+            async_call_cast<OP_TYPE>(0, promise);
+        }
     }
 
     template <typename BARRIER>
@@ -1278,6 +1289,11 @@ class AsyncProgram {
     }
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
+
+    // Promises can be held on a RemappingBarrier (futures may forward).
+    static constexpr bool REMAPPING_AWAIT = true;
+    // Records a misuse of the future API; complete() reports it.
+    __device__ void usage_error() { set_flags(USAGE_FLAG); }
 
     // One workgroup = GROUPS_PER_BLOCK independent groups (wavefronts).
     __device__ static HZ_INLINE void kernel_exec(DeviceContext ctx, DeviceState state, unsigned int cycles) {

@@ -20,6 +20,11 @@
 //    marks every partial word and the list head as released, closes partial
 //    links at their reserved count and moves the full list to the queues.
 //  - the host executors hold the promises in a side table.
+//
+// A semaphore decrement is a release and the decrement that reaches zero an
+// acquire (hz_fence_release / hz_fence_acquire, provided by the platform
+// layer), so whatever a decrementer wrote before (a future's value, say) is
+// visible to the promises the barrier releases.
 #pragma once
 
 #include "promise.h"
@@ -27,7 +32,9 @@
 template <typename OP_SET, typename ADR_TYPE>
 struct RemappingBarrier {
     using AdrType = ADR_TYPE;
-    using UnionType = PromiseUnion<OP_SET>;
+    // Indexed like the program's operation set, which has ReturnOp appended
+    // when the operations return values.
+    using UnionType = PromiseUnion<typename hz::ExecOpSet<OP_SET>::type>;
     static const size_t TYPE_COUNT = UnionType::Info::COUNT;
 
     TaggedSemaphore semaphore;
@@ -53,7 +60,7 @@ struct RemappingBarrier {
     // Holds `promise` until release; runs it at once if already released
     // (reference barrier.h:460-609).
     template <bool CAN_RELEASE = true, typename PROGRAM, typename OP_TYPE>
-    __device__ void await(PROGRAM program, Promise<OP_TYPE> promise) {
+    __device__ HZ_INLINE void await(PROGRAM program, Promise<OP_TYPE> promise) {
         static_assert(UnionType::template Lookup<OP_TYPE>::type::CONTAINED,
                       "the barrier cannot hold promises of this operation");
         program.barrier_await(*this, promise);
@@ -61,51 +68,83 @@ struct RemappingBarrier {
 
     // Releases the barrier regardless of its semaphore (barrier.h:614-621).
     template <typename PROGRAM>
-    __device__ void release(PROGRAM program) {
+    __device__ HZ_INLINE void release(PROGRAM program) {
+        hz_fence_release();
         unsigned int before = atomicExch(&semaphore.sem, 0u);
-        if (before != 0u) program.barrier_release(*this);
+        if (before != 0u) {
+            hz_fence_acquire();
+            program.barrier_release(*this);
+        }
     }
 
     // Adds `delta` (wrapping, so (unsigned)-1 decrements) and releases when
     // the semaphore reaches zero (barrier.h:624-629).
     template <typename PROGRAM>
-    __device__ void add_semaphore(PROGRAM program, unsigned int delta) {
+    __device__ HZ_INLINE void add_semaphore(PROGRAM program, unsigned int delta) {
+        hz_fence_release();
         unsigned int before = atomicAdd(&semaphore.sem, delta);
-        if (before + delta == 0u) program.barrier_release(*this);
+        if (before + delta == 0u) {
+            hz_fence_acquire();
+            program.barrier_release(*this);
+        }
     }
 
     template <typename PROGRAM>
-    __device__ void sub_semaphore(PROGRAM program, unsigned int delta) {
+    __device__ HZ_INLINE void sub_semaphore(PROGRAM program, unsigned int delta) {
         add_semaphore(program, 0u - delta);
     }
 };
 
-// A barrier for one promise (reference barrier.h:648-699).
+// A barrier for one promise (reference barrier.h:648-699): when its
+// semaphore reaches zero the promise it was built with is queued. A unit
+// barrier built without a promise only counts (a future's readiness).
 template <typename OP_SET, typename ADR_TYPE>
 struct UnitBarrier {
     using AdrType = ADR_TYPE;
+    // The program's operation set (ReturnOp appended for value-returning
+    // operations), so that every UnitBarrier<S> of one specification has the
+    // same layout whichever of the two sets S names.
+    using EnumType = PromiseEnum<typename hz::ExecOpSet<OP_SET>::type>;
+    static constexpr OpDisc NO_PROMISE = 0xFFFFFFFFu;
 
     TaggedSemaphore semaphore;
-    PromiseEnum<OP_SET> promise;
+    EnumType promise;
 
     UnitBarrier() = default;
-    HZ_HD UnitBarrier(PromiseEnum<OP_SET> promise_value, unsigned int semaphore_value)
+    HZ_HD UnitBarrier(EnumType promise_value, unsigned int semaphore_value)
         : semaphore(semaphore_value, hz::UNIT_BARRIER_TAG), promise(promise_value) {}
+    template <typename OP_TYPE>
+    HZ_HD UnitBarrier(Promise<OP_TYPE> promise_value, unsigned int semaphore_value)
+        : semaphore(semaphore_value, hz::UNIT_BARRIER_TAG), promise(promise_value) {}
+    explicit HZ_HD UnitBarrier(unsigned int semaphore_value) : semaphore(semaphore_value, hz::UNIT_BARRIER_TAG) {
+        promise.disc = NO_PROMISE;
+    }
 
     template <typename PROGRAM>
-    __device__ void release(PROGRAM program) {
+    __device__ HZ_INLINE void release(PROGRAM program) {
+        hz_fence_release();
         unsigned int old_val = atomicExch(&semaphore.sem, 0u);
-        if (old_val != 0u) program.async_call(promise);
+        if (old_val != 0u) fire(program);
     }
 
     template <typename PROGRAM>
-    __device__ void add_semaphore(PROGRAM program, unsigned int delta) {
+    __device__ HZ_INLINE void add_semaphore(PROGRAM program, unsigned int delta) {
+        hz_fence_release();
         unsigned int old_val = atomicAdd(&semaphore.sem, delta);
-        if ((old_val + delta) == 0u && old_val != 0u) program.async_call(promise);
+        if ((old_val + delta) == 0u && old_val != 0u) fire(program);
     }
 
     template <typename PROGRAM>
-    __device__ void sub_semaphore(PROGRAM program, unsigned int delta) {
+    __device__ HZ_INLINE void sub_semaphore(PROGRAM program, unsigned int delta) {
         add_semaphore(program, 0u - delta);
     }
+
+   private:
+    template <typename PROGRAM>
+    __device__ HZ_INLINE void fire(PROGRAM program) {
+        hz_fence_acquire();
+        if (promise.disc != NO_PROMISE) program.async_call(promise);
+    }
 };
+
+#include "future.h"

@@ -38,7 +38,7 @@ class EventProgram {
     typedef EventProgram<PROGRAM_SPEC> ProgramType;
 
     using AdrType = typename hz::spec::AdrTypeOr<PROGRAM_SPEC, unsigned int>::type;
-    using OpSet = typename hz::spec::OpSetOr<PROGRAM_SPEC, OpUnion<>>::type;
+    using OpSet = hz::spec::ProgramOpSet<PROGRAM_SPEC>;
     using DeviceState = typename hz::spec::DeviceStateOr<PROGRAM_SPEC, VoidState>::type;
     using GroupState = typename hz::spec::GroupStateOr<PROGRAM_SPEC, VoidState>::type;
     using ThreadState = typename hz::spec::ThreadStateOr<PROGRAM_SPEC, VoidState>::type;
@@ -61,6 +61,7 @@ class EventProgram {
 
     static const unsigned int BAD_FUNC_ID_FLAG = 0x00000001;
     static const unsigned int OVERFLOW_FLAG = 0x00000004;
+    static const unsigned int USAGE_FLAG = 0x00000020;
     static const unsigned int EARLY_HALT_FLAG = 0x40000000;
     static const unsigned int COMPLETION_FLAG = 0x80000000;
 
@@ -171,6 +172,11 @@ class EventProgram {
             if (flags & OVERFLOW_FLAG) {
                 throw std::runtime_error(
                     "EventProgram: an output queue overflowed; construct the Instance with a larger io_size");
+            }
+            if (flags & USAGE_FLAG) {
+                throw std::runtime_error(
+                    "EventProgram: a Return forwarded a future that was not ready (EventProgram cannot hold "
+                    "promises on a barrier)");
             }
             return (flags & COMPLETION_FLAG) != 0;
         }
@@ -422,23 +428,30 @@ class EventProgram {
 
     __device__ void halt_early() { set_flags(EARLY_HALT_FLAG); }
 
-    // Queues a promise held as a PromiseEnum (UnitBarrier release).
-    __device__ void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
+    // Promises cannot be held on a RemappingBarrier (see barrier_await).
+    static constexpr bool REMAPPING_AWAIT = false;
+    // Records a misuse of the future API; complete() reports it.
+    __device__ void usage_error() { set_flags(USAGE_FLAG); }
 
-    // A RemappingBarrier needs the async program's link arena, as in the
-    // reference (sync/barrier.h uses AsyncProgram internals throughout).
+    // Queues a promise held as a PromiseEnum (UnitBarrier release).
+    __device__ HZ_INLINE void async_call(PromiseEnum<OpSet> promise) { async_enum<0>(promise); }
+
+    // Holding promises on a RemappingBarrier needs the async program's link
+    // arena, as in the reference (sync/barrier.h uses AsyncProgram internals
+    // throughout), so awaiting is rejected at compile time. Releasing one is
+    // allowed and has nothing to run: no promise can have been held. (A
+    // future backed by a RemappingBarrier still carries its value and its
+    // ready state under EventProgram.)
     template <typename BARRIER, typename OP_TYPE>
     __device__ void barrier_await(BARRIER&, Promise<OP_TYPE>) {
-        static_assert(sizeof(OP_TYPE) == 0, "RemappingBarrier is supported by AsyncProgram, not EventProgram");
+        static_assert(sizeof(OP_TYPE) == 0, "RemappingBarrier::await is supported by AsyncProgram, not EventProgram");
     }
     template <typename BARRIER>
-    __device__ void barrier_release(BARRIER&) {
-        static_assert(sizeof(BARRIER) == 0, "RemappingBarrier is supported by AsyncProgram, not EventProgram");
-    }
+    __device__ void barrier_release(BARRIER&) {}
 
    protected:
     template <unsigned int I>
-    __device__ void async_enum(PromiseEnum<OpSet>& pe) {
+    __device__ HZ_INLINE void async_enum(PromiseEnum<OpSet>& pe) {
         if constexpr (I < OPS) {
             using Op = typename hz::OpOf<I, OpSet>::type;
             if (pe.disc == I) {

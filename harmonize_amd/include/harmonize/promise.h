@@ -40,6 +40,16 @@ struct UnitBarrier;
 template <typename TYPE, typename BARRIER>
 struct Future;
 
+// Internal operation that fills in a future's value and resolves its barrier
+// (reference async/future.h:37-60); defined in future.h.
+struct ReturnOp;
+
+// What a value-returning operation's eval may return: a value, a promise of
+// an operation returning the same type, or the address of a future whose value
+// becomes the result (reference async/promise.h:350-434); defined in future.h.
+template <typename PROGRAM, typename TYPE>
+struct Return;
+
 namespace hz {
 constexpr unsigned int BARRIER_NO_LINK = 0xFFFFFFFFu;   // no link / empty list
 constexpr unsigned int BARRIER_RELEASED = 0xFFFFFFFEu;  // partial word or list head after release
@@ -131,6 +141,16 @@ struct TaggedSemaphore {
     HZ_HD TaggedSemaphore(unsigned int sem_val, unsigned int tag_val) : sem(sem_val), tag(tag_val) {}
 };
 
+// Internal operation that resolves a future: it copies `size` bytes of the
+// value from `src` to `dst` and then decrements the barrier whose semaphore
+// is `sem` (reference async/future.h:37-60). Program types add it to the
+// operation set of a specification that has value-returning operations.
+struct ReturnOp {
+    using Type = void (*)(TaggedSemaphore*, void*, void*, size_t);
+    template <typename PROGRAM>
+    __device__ HZ_INLINE static void eval(PROGRAM program, TaggedSemaphore* sem, void* dst, void* src, size_t size);
+};
+
 // Marks an argument that the reference loads lazily (async/promise.h:204-240).
 // Here the value is read eagerly when the promise is built.
 template <typename TYPE>
@@ -202,6 +222,14 @@ struct ReturnAdr<void> {
 };
 
 namespace hz {
+
+// Delivers the result of a value-returning operation to its return address:
+// a value is stored and the address's barrier decremented; a returned promise
+// inherits the address and is queued; a returned future forwards its value
+// when it resolves. Defined in future.h.
+template <typename PROGRAM, typename TYPE, typename RESULT>
+__device__ __host__ HZ_INLINE void resolve(PROGRAM program, ReturnAdr<TYPE> adr, RESULT result);
+
 namespace detail {
 
 // Void operations carry no return address at all (empty base).
@@ -217,25 +245,22 @@ struct Unpack;
 
 template <typename OP, typename PROGRAM, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<>, UNROLLED...> {
-    template <typename R>
-    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<>&, UNROLLED... u) {
+    __device__ __host__ HZ_INLINE static decltype(auto) call(PROGRAM prog, const ArgTuple<>&, UNROLLED... u) {
         return OP::template eval<PROGRAM>(prog, u...);
     }
 };
 
 template <typename OP, typename PROGRAM, typename H, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<H>, UNROLLED...> {
-    template <typename R>
-    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<H>& t, UNROLLED... u) {
+    __device__ __host__ HZ_INLINE static decltype(auto) call(PROGRAM prog, const ArgTuple<H>& t, UNROLLED... u) {
         return OP::template eval<PROGRAM>(prog, u..., t.head);
     }
 };
 
 template <typename OP, typename PROGRAM, typename H, typename N, typename... T, typename... UNROLLED>
 struct Unpack<OP, PROGRAM, ArgTuple<H, N, T...>, UNROLLED...> {
-    template <typename R>
-    __device__ __host__ HZ_INLINE static R call(PROGRAM prog, const ArgTuple<H, N, T...>& t, UNROLLED... u) {
-        return Unpack<OP, PROGRAM, ArgTuple<N, T...>, UNROLLED..., H>::template call<R>(prog, t.tail, u..., t.head);
+    __device__ __host__ HZ_INLINE static decltype(auto) call(PROGRAM prog, const ArgTuple<H, N, T...>& t, UNROLLED... u) {
+        return Unpack<OP, PROGRAM, ArgTuple<N, T...>, UNROLLED..., H>::call(prog, t.tail, u..., t.head);
     }
 };
 
@@ -251,13 +276,30 @@ struct Promise : hz::detail::PromiseReturn<typename OpType<typename OPERATION::T
 
     Promise() = default;
 
+    // A promise of a value-returning operation starts with a null return
+    // address (its result is dropped); set `ret` (Future::adr() gives one) to
+    // deliver the result. `Promise<Op>()` value-initializes, so it is null too.
     template <typename... ARGS>
-    HZ_HD Promise(ARGS... a) : args(a...) {}
+    HZ_HD Promise(ARGS... a) : args(a...) {
+        if constexpr (!std::is_void<Return>::value) this->ret = ReturnAdr<Return>{nullptr, nullptr};
+    }
 
-    // Evaluates the operation with the stored arguments on the given program.
+    // Evaluates the operation with the stored arguments on the given program
+    // and returns what its eval returns.
     template <typename PROGRAM>
-    __device__ __host__ HZ_INLINE Return operator()(PROGRAM program) {
-        return hz::detail::Unpack<OPERATION, PROGRAM, Args>::template call<Return>(program, args);
+    __device__ __host__ HZ_INLINE decltype(auto) operator()(PROGRAM program) {
+        return hz::detail::Unpack<OPERATION, PROGRAM, Args>::call(program, args);
+    }
+
+    // Evaluates the promise the way the program types do: the result of a
+    // value-returning operation goes to `ret` (hz::resolve, future.h).
+    template <typename PROGRAM>
+    __device__ __host__ HZ_INLINE void run(PROGRAM program) {
+        if constexpr (std::is_void<Return>::value) {
+            (*this)(program);
+        } else {
+            hz::resolve(program, this->ret, (*this)(program));
+        }
     }
 };
 
@@ -277,6 +319,8 @@ union PromiseUnion<OpUnion<>> {
     template <typename PROGRAM>
     __device__ __host__ void loose_eval(PROGRAM, OpDisc) {}
     HZ_HD void dyn_copy_as(OpDisc, PromiseUnion<OpUnion<>>&) {}
+    template <typename TYPE>
+    HZ_HD void dyn_set_return(OpDisc, ReturnAdr<TYPE>) {}
     PromiseUnion() = default;
 };
 
@@ -324,14 +368,14 @@ union PromiseUnion<OpUnion<HEAD, TAIL...>> {
     // Evaluates the union as a promise of the statically known operation TYPE.
     template <typename PROGRAM, typename TYPE>
     __device__ __host__ HZ_INLINE void rigid_eval(PROGRAM program) {
-        cast<TYPE>()(program);
+        cast<TYPE>().run(program);
     }
 
     // Evaluates the union as a promise of the operation with discriminant `disc`.
     template <typename PROGRAM>
     __device__ __host__ HZ_INLINE void loose_eval(PROGRAM program, OpDisc disc) {
         if (disc == 0) {
-            head_form(program);
+            head_form.run(program);
         } else {
             if constexpr (sizeof...(TAIL) > 0) {
                 tail_form.template loose_eval<PROGRAM>(program, disc - 1);
@@ -349,6 +393,17 @@ union PromiseUnion<OpUnion<HEAD, TAIL...>> {
             head_form = other.head_form;
         } else {
             tail_form.dyn_copy_as(disc - 1, other.tail_form);
+        }
+    }
+
+    // Sets the return address of the member with discriminant `disc`, which
+    // must be a promise of an operation returning TYPE.
+    template <typename TYPE>
+    HZ_HD void dyn_set_return(OpDisc disc, ReturnAdr<TYPE> adr) {
+        if (disc == 0) {
+            if constexpr (std::is_same<typename Head::Return, TYPE>::value) head_form.ret = adr;
+        } else {
+            if constexpr (sizeof...(TAIL) > 0) tail_form.dyn_set_return(disc - 1, adr);
         }
     }
 };
@@ -374,6 +429,21 @@ namespace hz {
 template <typename OP_SET>
 struct PromiseWords {
     static constexpr uint32_t value = WordCount<PromiseUnion<OP_SET>>::value;
+};
+
+// The operation set a program type executes for a specification's OpSet:
+// ReturnOp is appended when some operation returns a value (futures are
+// resolved by ReturnOp promises held on barriers) and is not already there.
+// Specifications whose operations all return void keep their set unchanged.
+template <typename OP_SET>
+struct ExecOpSet;
+template <typename... OPS>
+struct ExecOpSet<OpUnion<OPS...>> {
+    static constexpr bool returns_values =
+        (false || ... || !std::is_void<typename OpType<typename OPS::Type>::Return>::value);
+    static constexpr bool has_return_op = (false || ... || std::is_same<OPS, ReturnOp>::value);
+    using type = typename std::conditional<returns_values && !has_return_op, OpUnion<OPS..., ReturnOp>,
+                                           OpUnion<OPS...>>::type;
 };
 
 template <typename OP_SET>

@@ -54,5 +54,10 @@ HZ_CONST_OR(GROUPS_PER_BLOCK)
 HZ_CONST_OR(SHARE_WORK)
 #undef HZ_CONST_OR
 
+// The operation set a program type executes: the specification's OpSet, plus
+// ReturnOp when an operation returns a value (hz::ExecOpSet).
+template <typename S>
+using ProgramOpSet = typename ExecOpSet<typename OpSetOr<S, OpUnion<>>::type>::type;
+
 }  // namespace spec
 }  // namespace hz

@@ -125,6 +125,11 @@ __device__ HZ_INLINE unsigned int hz_group_index() { return blockIdx.x; }
 __device__ HZ_INLINE unsigned int hz_block_thread() { return threadIdx.x; }
 __device__ HZ_INLINE unsigned int hz_block_threads() { return blockDim.x; }
 
+// Platform hooks used by the portable barrier layer: device-wide release and
+// acquire ordering around semaphore updates.
+__device__ HZ_INLINE void hz_fence_release() { hz::wave::fence_release(); }
+__device__ HZ_INLINE void hz_fence_acquire() { hz::wave::fence_acquire(); }
+
 // Platform hooks used by the portable iterator layer.
 template <typename T>
 __device__ HZ_INLINE T hz_fetch_add(T* counter, T amount) {

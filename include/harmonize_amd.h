@@ -283,6 +283,22 @@ int hz_detmath_eval(const double* d_in, double* d_log, double* d_sqrt, uint32_t*
 int hz_barrier_run(uint32_t dep_count, uint32_t non_dep_count, uint32_t barrier_count, uint32_t* d_counter,
                    uint32_t* d_hits, uint32_t* d_violations, const hz_run_config* config, hz_run_stats* stats);
 
+/* Runs the futures program: value-returning operations resolving a binary
+ * tree of futures over Collatz step counts (no reference counterpart: the
+ * reference declares Future, UnitBarrier, ReturnOp and Return,
+ * async/future.h:7-60, sync/barrier.h:648-699, async/promise.h:326-434,
+ * but leaves their resolution unfinished).
+ *   leaves   power of two; leaf i computes steps(d_starts[i]) through a chain
+ *            of promises that returns the count;
+ *   *total   the sum of all leaves' counts, resolved through the tree;
+ *   readers  (HZ_ASYNC only) reader r awaits the total on its
+ *            RemappingBarrier future and writes d_echo[r] = total + r, and
+ *            forwards the future itself: d_forwarded[r] = total. A reader
+ *            that ran early, or a future left unresolved, shows as
+ *            UINT64_MAX. */
+int hz_futures_run(int kind, const uint64_t* d_starts, uint32_t leaves, uint32_t readers, uint64_t* d_echo,
+                   uint64_t* d_forwarded, uint64_t* total, const hz_run_config* config, hz_run_stats* stats);
+
 /* ------------------------------------------------------- batched BFS
  * A batch owns the CSR graphs of many jobs, their level arrays, and the
  * program instances that run them (kept between runs, so a run with the same

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <deque>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 #include <unordered_map>
 #include <utility>
@@ -50,7 +51,7 @@ template <typename SPEC>
 class HostProgram {
    public:
     typedef HostProgram<SPEC> ProgramType;
-    using OpSet = typename hz::spec::OpSetOr<SPEC, OpUnion<>>::type;
+    using OpSet = hz::spec::ProgramOpSet<SPEC>;
     using DeviceState = typename hz::spec::DeviceStateOr<SPEC, VoidState>::type;
     using GroupState = typename hz::spec::GroupStateOr<SPEC, VoidState>::type;
     using ThreadState = typename hz::spec::ThreadStateOr<SPEC, VoidState>::type;
@@ -110,6 +111,10 @@ class HostProgram {
     // A GPU pass halted early is resumed by the host loop; the sequential
     // executor therefore just records the request and carries on.
     void halt_early() { _run.stats.halts++; }
+
+    // Promises can be held on a RemappingBarrier (futures may forward).
+    static constexpr bool REMAPPING_AWAIT = true;
+    void usage_error() { throw std::logic_error("HostProgram: a Return forwarded a unit future that was not ready"); }
 
     void async_call(PromiseEnum<OpSet> pe) {
         _run.queue.emplace_back(pe.disc, pe.data);

@@ -32,6 +32,11 @@ inline unsigned int hz_block_threads() { return 1u; }
 inline void __syncthreads() {}
 #define __shared__ static
 
+// Ordering around barrier semaphore updates (the atomics below are
+// sequentially consistent already; the fences keep plain stores ordered).
+inline void hz_fence_release() { __atomic_thread_fence(__ATOMIC_RELEASE); }
+inline void hz_fence_acquire() { __atomic_thread_fence(__ATOMIC_ACQUIRE); }
+
 template <typename T>
 inline T hz_fetch_add(T* counter, T amount) {
     return __atomic_fetch_add(counter, amount, __ATOMIC_SEQ_CST);

@@ -12,6 +12,7 @@
 #include "collatz.h"
 #include "photon.h"
 #include "barrier_demo.h"
+#include "futures.h"
 #include "bfs.h"
 #include "philox.h"
 #include "rmat.h"
@@ -211,6 +212,43 @@ int oracle_barrier_program(uint32_t dep, uint32_t non_dep, uint32_t barrier_coun
     oracle::RunStats s = used == 1 ? oracle::HostProgram<BarrierDemoSpec>::run(st)
                                    : oracle::OmpProgram<BarrierDemoSpec>::run(st, used);
     auto t1 = std::chrono::steady_clock::now();
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Runs the futures specification on the host executors (same outputs as
+// hz_futures_run). Unresolved futures read as UINT64_MAX.
+int oracle_futures_program(const uint64_t* starts, uint32_t leaves, uint32_t readers, uint64_t* echo,
+                           uint64_t* forwarded, uint64_t* total, int threads, oracle_run_stats* stats) {
+    if (leaves == 0 || (leaves & (leaves - 1u)) != 0) return -1;
+    using TotalFuture = Future<unsigned long long, RemappingBarrier<FuturesOps>>;
+    using UnitFuture = Future<unsigned long long, UnitBarrier<FuturesOps>>;
+    std::vector<unsigned long long> slot((size_t)leaves * 2u, 0ull);
+    std::vector<UnitBarrier<FuturesOps>> join(leaves);
+    TotalFuture tot(1u);
+    std::vector<UnitFuture> fwd(readers ? readers : 1u);
+    util::iter::AtomicIter<unsigned int> iter(0u, leaves + readers);
+    FuturesState st{};
+    st.starts = (const unsigned long long*)starts;
+    st.slot = slot.data();
+    st.join = join.data();
+    st.total = &tot;
+    st.forwarded = fwd.data();
+    st.echo = (unsigned long long*)echo;
+    st.iter = &iter;
+    st.leaves = leaves;
+    st.readers = readers;
+    uint32_t setup = leaves > readers ? leaves : readers;
+    for (uint32_t k = 0; k < setup; ++k) futures_setup_node(st, k);
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s = used == 1 ? oracle::HostProgram<FuturesSpec>::run(st)
+                                   : oracle::OmpProgram<FuturesSpec>::run(st, used);
+    auto t1 = std::chrono::steady_clock::now();
+    *total = tot.barrier.semaphore.sem == 0u ? tot.data : FUTURES_EARLY;
+    for (uint32_t r = 0; r < readers; ++r) {
+        forwarded[r] = fwd[r].barrier.semaphore.sem == 0u ? fwd[r].data : FUTURES_EARLY;
+    }
     fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
     return 0;
 }

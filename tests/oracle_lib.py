@@ -124,6 +124,27 @@ def barrier_program(dep, non_dep, barriers, threads=1):
     return int(counter[0]), hits[:dep], int(violations[0]), st
 
 
+_SIGS["oracle_futures_program"] = (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int,
+                                                   ctypes.POINTER(OracleStats)])
+
+FUTURES_EARLY = (1 << 64) - 1
+
+
+def futures_program(starts, readers, threads=1):
+    """Runs the futures specification on the host executors over `starts` (a
+    power-of-two count); returns (total, echo[readers], forwarded[readers], stats)."""
+    starts = np.ascontiguousarray(starts, dtype=np.uint64)
+    echo = np.zeros(max(1, readers), dtype=np.uint64)
+    forwarded = np.zeros(max(1, readers), dtype=np.uint64)
+    total = np.zeros(1, dtype=np.uint64)
+    st = OracleStats()
+    rc = lib().oracle_futures_program(starts.ctypes.data, len(starts), readers, echo.ctypes.data,
+                                      forwarded.ctypes.data, total.ctypes.data, threads, ctypes.byref(st))
+    if rc != 0:
+        raise ValueError("futures_program: the leaf count must be a power of two")
+    return int(total[0]), echo[:readers], forwarded[:readers], st
+
+
 def photon_params(**overrides):
     d = dict(PHOTON_DEFAULTS)
     d.update(overrides)
