@@ -177,6 +177,69 @@ HZ_DECLARE_PROGRAM_ABI(bfs_event)
 HZ_DECLARE_PROGRAM_ABI(bfs_async)
 HZ_DECLARE_PROGRAM_ABI(barrier_async)
 
+/* ------------------------------- per-label state accessors (reference names)
+ * The reference's binding (harmonize/python/runtime.py:1348-1371) binds, for
+ * every state label of a program, load_state_<label>_<suffix>,
+ * store_state_<label>_<suffix> and store_pointer_state_<label>_<suffix>
+ * (templates.py:85-152). Labels are "device", "group" and "thread", and
+ * "device_<field>" for a device state given as fields (runtime.py:562-595).
+ * Besides the whole-state device accessors above, this library exports:
+ *
+ *   void load_state_device_<field>_<suffix>(void* host, void* dev_state);
+ *   void store_state_device_<field>_<suffix>(void* dev_state, void* host);
+ *       Copy one field (its size, at its offset) of the device state.
+ *   void store_pointer_state_device_<field>_<suffix>(void* dev_state, void* ptr);
+ *       Store the pointer value `ptr` into a pointer field; for a non-pointer
+ *       field it records the reference's error, "Attempted to set
+ *       indirection of non-array state field." (templates.py:146-150).
+ *   void store_pointer_state_device_<suffix>(void* dev_state, void* ptr);
+ *       The whole state is a record, not an array field: records that error.
+ *   void load_state_group_<suffix>, store_state_group_<suffix>,
+ *        load_state_thread_<suffix>, store_state_thread_<suffix>
+ *       (same argument order as the device ones): the built-in programs keep
+ *       no host-visible group or thread state, so these copy zero bytes;
+ *       store_pointer_state_group_<suffix> and _thread_<suffix> record the
+ *       error above.
+ *
+ * Fields (1 = pointer field): */
+#define HZ_COLLATZ_STATE_FIELDS(X, suffix) X(suffix, starts, 1) X(suffix, steps, 1) X(suffix, iterator, 1)
+#define HZ_PHOTON_STATE_FIELDS(X, suffix)                                                        \
+    X(suffix, photon_count, 0) X(suffix, seed_lo, 0) X(suffix, seed_hi, 0) X(suffix, bins, 0)    \
+    X(suffix, replicas, 0) X(suffix, max_split, 0) X(suffix, max_events, 0) X(suffix, regions, 0) \
+    X(suffix, thickness, 0) X(suffix, mean_free_path, 0) X(suffix, albedo, 0) X(suffix, bin_scale, 0) \
+    X(suffix, region_scale, 0) X(suffix, split_above, 0) X(suffix, roulette_below, 0)            \
+    X(suffix, tally, 1) X(suffix, events, 1) X(suffix, iterator, 1)
+#define HZ_BFS_STATE_FIELDS(X, suffix) X(suffix, jobs, 1) X(suffix, job_iter, 1) X(suffix, job_count, 0) X(suffix, split, 0)
+#define HZ_BARRIER_STATE_FIELDS(X, suffix)                                                       \
+    X(suffix, dep_count, 0) X(suffix, non_dep_count, 0) X(suffix, barrier_count, 0)             \
+    X(suffix, source_id_iter, 1) X(suffix, counter, 1) X(suffix, hits, 1) X(suffix, violations, 1) X(suffix, barrier, 1)
+
+#define HZ_DECLARE_STATE_FIELD(suffix, field, is_pointer)                           \
+    void load_state_device_##field##_##suffix(void* host_ptr, void* dev_ptr);       \
+    void store_state_device_##field##_##suffix(void* dev_ptr, void* host_ptr);      \
+    void store_pointer_state_device_##field##_##suffix(void* dev_ptr, void* host_ptr);
+#define HZ_DECLARE_STATE_LABELS(suffix, FIELDS)                                     \
+    FIELDS(HZ_DECLARE_STATE_FIELD, suffix)                                          \
+    void store_pointer_state_device_##suffix(void* dev_ptr, void* host_ptr);        \
+    void load_state_group_##suffix(void* host_ptr, void* dev_ptr);                  \
+    void store_state_group_##suffix(void* dev_ptr, void* host_ptr);                 \
+    void store_pointer_state_group_##suffix(void* dev_ptr, void* host_ptr);         \
+    void load_state_thread_##suffix(void* host_ptr, void* dev_ptr);                 \
+    void store_state_thread_##suffix(void* dev_ptr, void* host_ptr);                \
+    void store_pointer_state_thread_##suffix(void* dev_ptr, void* host_ptr);
+
+HZ_DECLARE_STATE_LABELS(collatz_event, HZ_COLLATZ_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(collatz_async, HZ_COLLATZ_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(photon_event, HZ_PHOTON_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(photon_async, HZ_PHOTON_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(bfs_event, HZ_BFS_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(bfs_async, HZ_BFS_STATE_FIELDS)
+HZ_DECLARE_STATE_LABELS(barrier_async, HZ_BARRIER_STATE_FIELDS)
+
+/* The device's wall-clock rate in Hz (templates.py:268-285), for converting
+ * wall-clock readings taken inside specifications; 0 on error. */
+long long wall_clock_rate(void);
+
 /* -------------------------------------------- generic instances (layer 1)
  * One handle for any built-in specification and kind, chosen at run time
  * (the boundary the per-program names above expand to):

@@ -19,15 +19,57 @@ PROGRAM_ENTRY_POINTS = [
 ]
 
 
+STATE_LABEL_ENTRY_POINTS = [
+    "store_pointer_state_device_{}", "load_state_group_{}", "store_state_group_{}", "store_pointer_state_group_{}",
+    "load_state_thread_{}", "store_state_thread_{}", "store_pointer_state_thread_{}",
+]
+FIELD_ENTRY_POINTS = ["load_state_device_{f}_{s}", "store_state_device_{f}_{s}", "store_pointer_state_device_{f}_{s}"]
+
+
+def state_field_lists(text):
+    """{list macro name: [(field, is_pointer)]} from the header's X-macro field lists."""
+    lists = {}
+    for name, body in re.findall(r"#define (HZ_\w+_STATE_FIELDS)\(X, suffix\)((?:.*\\\n)*.*)\n", text):
+        lists[name] = [(f, p == "1") for f, p in re.findall(r"X\(suffix, (\w+), ([01])\)", body)]
+    return lists
+
+
 def declared_symbols():
     text = HEADER.read_text()
-    # Drop comments and the body of the per-program declaration macro.
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    text = re.sub(r"#define HZ_DECLARE_PROGRAM_ABI\(suffix\)(?:.*\\\n)*.*\n", "", text)
+    lists = state_field_lists(text)
+    # Drop macro definitions (their bodies are expanded below).
+    text = re.sub(r"#define \w+\([^)]*\)(?:.*\\\n)*.*\n", "", text)
     names = set(re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, flags=re.M))
     for suffix in re.findall(r"^HZ_DECLARE_PROGRAM_ABI\((\w+)\)", text, flags=re.M):
         names.update(p.format(suffix) for p in PROGRAM_ENTRY_POINTS)
+    for suffix, fields in re.findall(r"^HZ_DECLARE_STATE_LABELS\((\w+), (\w+)\)", text, flags=re.M):
+        names.update(p.format(suffix) for p in STATE_LABEL_ENTRY_POINTS)
+        for field, _ in lists[fields]:
+            names.update(p.format(f=field, s=suffix) for p in FIELD_ENTRY_POINTS)
     return sorted(names)
+
+
+def test_state_field_lists_match_the_c_structs():
+    # Every listed field exists in its struct, in declaration order, and the
+    # pointer flags agree with the declared types.
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    lists = state_field_lists(text)
+    structs = {"HZ_COLLATZ_STATE_FIELDS": "hz_collatz_state", "HZ_PHOTON_STATE_FIELDS": "hz_photon_state",
+               "HZ_BFS_STATE_FIELDS": "hz_bfs_state", "HZ_BARRIER_STATE_FIELDS": "hz_barrier_state"}
+    assert set(lists) == set(structs)
+    for macro, struct in structs.items():
+        body = re.search(r"typedef struct %s \{(.*?)\}" % struct, text, flags=re.S).group(1)
+        decls = []
+        for line in body.split(";"):
+            line = line.strip()
+            if not line:
+                continue
+            ptr = "*" in line
+            for name in re.findall(r"\*?\s*(\w+)\s*(?:,|$)", line.split(None, 1)[1] if " " in line else line):
+                if name != "pad":
+                    decls.append((name, ptr))
+        assert lists[macro] == decls, macro
 
 
 def test_header_parses_to_expected_surface():
@@ -36,6 +78,9 @@ def test_header_parses_to_expected_surface():
                  "alloc_program_collatz_event"):
         assert must in names
     assert len(names) >= 30
+    for must in ("load_state_device_starts_collatz_async", "store_pointer_state_device_tally_photon_event",
+                 "store_state_device_split_bfs_async", "load_state_group_barrier_async", "wall_clock_rate"):
+        assert must in names
 
 
 def test_library_exports_every_declared_symbol():
