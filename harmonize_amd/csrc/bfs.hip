@@ -253,7 +253,9 @@ void run_event(hz_bfs_batch& b, const hz_run_config& c, hz_run_stats* stats) {
         if (io > (1ull << 30)) io = 1ull << 30;
     }
     BfsEvent::Instance inst(io, b.state(default_split(c)));
-    uint32_t groups = c.groups ? c.groups : (uint32_t)(hz::capi::compute_units() * 32);
+    // 16 groups per CU: measured 22.2 ms on the 64-graph batch against 23.7
+    // with 32 per CU and 23.9 with 8 per CU.
+    uint32_t groups = c.groups ? c.groups : (uint32_t)(hz::capi::compute_units() * 16);
     uint32_t chunk = c.cycles ? c.cycles : 4u;
     uint32_t max_launches = c.max_launches ? c.max_launches : (1u << 24);
     StreamTimer t(b.stream);
