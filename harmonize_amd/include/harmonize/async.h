@@ -573,7 +573,7 @@ class AsyncProgram {
             fm = hz::wave::first(g.free_mask);
         }
         uint32_t L = lowest(fm);
-        if (hz::wave::leader()) g.free_mask = fm & ~(1u << L);
+        g.free_mask = fm & ~(1u << L);
         hz::wave::sync();
         return L;
     }
@@ -633,31 +633,33 @@ class AsyncProgram {
         if (L == hz::LINK_NONE) {
             L = take_link();
             if (L == hz::LINK_NONE) return;
-            if (r == 0) {
-                g.link_op[L] = D;
-                g.link_count[L] = 0;
-                g.open[D] = L;
-            }
+            g.link_op[L] = D;
+            g.link_count[L] = 0;
+            g.open[D] = L;
             hz::wave::sync();
         } else {
             c = hz::wave::first(g.link_count[L]);
         }
         uint32_t room = 64u - c;
+        if (n < room) {
+            // The common case, decided by a wave-uniform test: every active
+            // lane writes its promise and the (uniform) new count, with no
+            // per-lane predication for the scalar unit to set up.
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
+            g.link_count[L] = c + n;
+            hz::wave::sync();
+            return;
+        }
         if (r < room) {
 #pragma unroll
             for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
         }
-        if (n < room) {
-            if (r == 0) g.link_count[L] = c + n;
-            hz::wave::sync();
-            return;
-        }
         // L is now full: close it, then open another for the overflow.
-        if (r == 0) {
-            g.link_count[L] = 64u;
-            g.ready_mask |= (1u << L);
-            g.open[D] = hz::LINK_NONE;
-        }
+        uint32_t rmask = hz::wave::first(g.ready_mask);
+        g.link_count[L] = 64u;
+        g.ready_mask = rmask | (1u << L);
+        g.open[D] = hz::LINK_NONE;
         hz::wave::sync();
         uint32_t rest = n - room;
         if (rest == 0) return;
@@ -667,11 +669,9 @@ class AsyncProgram {
 #pragma unroll
             for (uint32_t w = 0; w < WORDS; ++w) g.words[L2][w][r - room] = words[w];
         }
-        if (r == 0) {
-            g.link_op[L2] = D;
-            g.link_count[L2] = rest;
-            g.open[D] = L2;
-        }
+        g.link_op[L2] = D;
+        g.link_count[L2] = rest;
+        g.open[D] = L2;
         hz::wave::sync();
     }
 
@@ -759,12 +759,14 @@ class AsyncProgram {
                 for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
             }
             hz::wave::sync();
-            if (lane == 0) {
-                g.free_mask |= (1u << L);
-                g.ready_mask &= ~(1u << L);
-                g.executed += cnt;
-                g.runs += 1;
-            }
+            // All lanes store the same values (uniform read-modify-writes of
+            // one wavefront): no lane-0 predication on the hot path.
+            uint32_t fm = hz::wave::first(g.free_mask);
+            uint32_t rmask = hz::wave::first(g.ready_mask);
+            g.free_mask = fm | (1u << L);
+            g.ready_mask = rmask & ~(1u << L);
+            g.executed += cnt;
+            g.runs += 1;
             hz::wave::sync();
         }
         {
@@ -836,7 +838,7 @@ class AsyncProgram {
                 }
                 if (L != hz::LINK_NONE) {
                     uint32_t op = hz::wave::first(g.link_op[L]);
-                    if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
+                    g.open[op] = hz::LINK_NONE;
                     hz::wave::sync();
                 }
             }
@@ -865,7 +867,7 @@ class AsyncProgram {
             if (!check) {
                 uint32_t countdown = hz::wave::first(g.queue_check);
                 check = countdown == 0;
-                if (hz::wave::lane() == 0) g.queue_check = check ? QUEUE_CHECK_PERIOD - 1 : countdown - 1;
+                g.queue_check = check ? QUEUE_CHECK_PERIOD - 1 : countdown - 1;
                 hz::wave::sync();
             }
             if (check) {
@@ -873,7 +875,7 @@ class AsyncProgram {
                 if (queues_hint()) {
                     pull_links(1);
                     rm = hz::wave::first(g.ready_mask);
-                    if (hz::wave::lane() == 0) g.queue_check = 0;  // keep draining while links remain
+                    g.queue_check = 0;  // keep draining while links remain
                     hz::wave::sync();
                 }
             }
@@ -898,14 +900,14 @@ class AsyncProgram {
                 L = fullest_open(c);
                 if (L != hz::LINK_NONE) {
                     uint32_t op = hz::wave::first(g.link_op[L]);
-                    if (hz::wave::lane() == 0) g.open[op] = hz::LINK_NONE;
+                    g.open[op] = hz::LINK_NONE;
                     hz::wave::sync();
                 }
             }
         }
 
         if (L != hz::LINK_NONE) {
-            if (hz::wave::lane() == 0) g.idle_streak = 0;
+            g.idle_streak = 0;
             set_hungry(false);
             share_links(L);
             run_link(L);
