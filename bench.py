@@ -113,6 +113,45 @@ def photon_comparison(hz, repeats: int = 3) -> dict:
     }
 
 
+FUTURES_LEAVES = 1 << 20
+FUTURES_READERS = 1 << 14
+
+
+def futures_comparison(hz, repeats: int = 3) -> dict:
+    """Value-returning operations: a tree of 2^20 unit futures summing the Collatz
+    step counts of 1..2^20 (each leaf a tail-called chain of Steps promises),
+    resolving a RemappingBarrier future that 2^14 readers await and forward.
+    EventProgram runs the tree only (it cannot hold promises on a barrier)."""
+    starts = hz.DeviceArray.from_numpy(np.arange(1, FUTURES_LEAVES + 1, dtype=np.uint64))
+    expected_total = 139_348_407 - FUTURES_LEAVES  # the reference checker's sum of (steps + 1) over 1..2^20
+
+    def best(kind, readers):
+        runs = []
+        for _ in range(repeats):
+            total, echo, forwarded, st = hz.futures_run(kind, starts, readers)
+            if total != expected_total or (readers and (int(echo[-1]) != total + readers - 1 or
+                                                        int(forwarded.max()) != total)):
+                raise RuntimeError(f"futures ({kind}): wrong result {total}")
+            runs.append(st)
+        return min(runs, key=lambda r: r["elapsed_ms"])
+
+    a = best("async", FUTURES_READERS)
+    a_tree = best("async", 0)
+    e = best("event", 0)
+    rate = lambda r: r["promises"] / (r["elapsed_ms"] * 1e-3)
+    return {
+        "workload": "futures tree: 2^20 leaves (Collatz steps of 1..2^20 through tail-called value-returning "
+                    "promises), unit-future joins, a RemappingBarrier total awaited and forwarded by 2^14 readers",
+        "async": {"value": rate(a), "unit": "promises/s", "ms": a["elapsed_ms"], "promises": a["promises"],
+                  "readers": FUTURES_READERS},
+        "async_tree_only": {"value": rate(a_tree), "unit": "promises/s", "ms": a_tree["elapsed_ms"],
+                            "promises": a_tree["promises"]},
+        "event_tree_only": {"value": rate(e), "unit": "promises/s", "ms": e["elapsed_ms"],
+                            "promises": e["promises"], "launches": e["launches"]},
+        "async_vs_event_tree": rate(a_tree) / rate(e),
+    }
+
+
 BFS_SEED = 0x5EED
 BFS_GRAPHS = 64
 
@@ -299,6 +338,7 @@ def main() -> int:
     ap.add_argument("--no-photon", action="store_true", help="skip the photon-walk Async vs Event comparison")
     ap.add_argument("--no-bfs", action="store_true", help="skip the batched BFS Async vs Event comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving load sweep")
+    ap.add_argument("--no-futures", action="store_true", help="skip the value-returning futures tree")
     ap.add_argument("--leg", choices=sorted(LEGS), help="run one leg once (for counter collection) and exit")
     args = ap.parse_args()
 
@@ -395,6 +435,9 @@ def main() -> int:
 
     if not args.no_bfs:
         out["bfs"] = bfs_comparison(hz)
+
+    if not args.no_futures:
+        out["futures"] = futures_comparison(hz)
 
     if not args.no_serve:
         out["serving"] = serving_sweep(hz)

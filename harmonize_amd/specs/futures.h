@@ -121,7 +121,10 @@ struct FuturesSpec {
     using OpSet = FuturesOps;
     using DeviceState = FuturesState;
 
-    static const size_t STASH_SIZE = 8;
+    // Six operations with ReturnOp: the async scheduler keeps OPS + 1 links
+    // free for a link's appends, so a stash of 8 would hold only two links of
+    // work and spill nearly every link it fills.
+    static const size_t STASH_SIZE = 16;
     static const size_t FRAME_SIZE = 8;
     static const size_t POOL_SIZE = 8;
 
