@@ -80,6 +80,13 @@ struct CollatzOdd {
 #ifndef HZ_COLLATZ_CLAIM
 #define HZ_COLLATZ_CLAIM 8
 #endif
+// Below this many unclaimed starts a thread claims one at a time, so that a
+// small job (or the end of a large one) spreads its chains over more
+// wavefronts: a chain advances once per scheduler cycle of its wavefront, so
+// a wavefront holding fewer chains finishes them sooner.
+#ifndef HZ_COLLATZ_SPREAD_BELOW
+#define HZ_COLLATZ_SPREAD_BELOW (1u << 20)
+#endif
 // Wavefront groups per workgroup.
 #ifndef HZ_COLLATZ_BLOCK
 #define HZ_COLLATZ_BLOCK 1
@@ -130,7 +137,10 @@ struct CollatzSpec {
     template <typename PROGRAM>
     __device__ static bool make_work(PROGRAM prog) {
         if (prog.thread.next >= prog.thread.end) {
-            util::iter::Iter<unsigned int> it = prog.device.iterator->leap(HZ_COLLATZ_CLAIM);
+            util::iter::AtomicIter<unsigned int>& starts = *prog.device.iterator;
+            unsigned int at = hz_load(&starts.value);
+            unsigned int left = at < starts.limit ? starts.limit - at : 0u;
+            util::iter::Iter<unsigned int> it = starts.leap(left < HZ_COLLATZ_SPREAD_BELOW ? 1u : HZ_COLLATZ_CLAIM);
             prog.thread.next = it.value;
             prog.thread.end = it.limit;
         }
