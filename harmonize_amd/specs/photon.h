@@ -264,6 +264,12 @@ struct PhotonStep {
 #ifndef HZ_PHOTON_CLAIM
 #define HZ_PHOTON_CLAIM 8
 #endif
+// Below this many unclaimed photons a thread claims one at a time, so a
+// small job (or the end of a large one) spreads its lineages over more
+// wavefronts.
+#ifndef HZ_PHOTON_SPREAD_BELOW
+#define HZ_PHOTON_SPREAD_BELOW (1u << 17)
+#endif
 #ifndef HZ_PHOTON_BLOCK
 #define HZ_PHOTON_BLOCK 4
 #endif
@@ -343,7 +349,10 @@ struct PhotonSpec {
     template <typename PROGRAM>
     __device__ static bool make_work(PROGRAM prog) {
         if (prog.thread.next >= prog.thread.end) {
-            util::iter::Iter<unsigned int> it = prog.device.iterator->leap(HZ_PHOTON_CLAIM);
+            util::iter::AtomicIter<unsigned int>& photons = *prog.device.iterator;
+            unsigned int at = hz_load(&photons.value);
+            unsigned int left = at < photons.limit ? photons.limit - at : 0u;
+            util::iter::Iter<unsigned int> it = photons.leap(left < HZ_PHOTON_SPREAD_BELOW ? 1u : HZ_PHOTON_CLAIM);
             prog.thread.next = it.value;
             prog.thread.end = it.limit;
         }
