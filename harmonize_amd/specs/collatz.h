@@ -85,7 +85,7 @@ struct CollatzOdd {
 // wavefronts: a chain advances once per scheduler cycle of its wavefront, so
 // a wavefront holding fewer chains finishes them sooner.
 #ifndef HZ_COLLATZ_SPREAD_BELOW
-#define HZ_COLLATZ_SPREAD_BELOW (1u << 20)
+#define HZ_COLLATZ_SPREAD_BELOW (1u << 22)
 #endif
 // Wavefront groups per workgroup.
 #ifndef HZ_COLLATZ_BLOCK
