@@ -50,7 +50,7 @@ def algorithmic_bytes(promises: int, starts: int) -> int:
 # reads for a 1 GiB device-to-device copy (`bench.py --leg copy`): FETCH_SIZE
 # reads 512 MiB for the copy's 1 GiB of reads (factor 2.0) and WRITE_SIZE
 # reads 1 GiB for its 1 GiB of writes (factor 1.0).
-TRAFFIC_KIB = {"fetch": 1095608.1, "write": 2353089.5}
+TRAFFIC_KIB = {"fetch": 1037835.9, "write": 2219237.9}
 TRAFFIC_FACTOR = {"fetch": 2.0, "write": 1.0}
 
 
