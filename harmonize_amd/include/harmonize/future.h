@@ -183,6 +183,31 @@ __device__ __host__ HZ_INLINE void fulfill(PROGRAM program, TaggedSemaphore* sem
     }
 }
 
+namespace detail {
+
+// Queues the promise held in `pe` with `adr` as its return address. Each
+// candidate operation is a branch on the discriminant that copies the promise
+// out as its own type, so no union member is written through a run-time
+// choice (which would keep the union in scratch memory).
+template <unsigned int I, typename PROGRAM, typename ENUM, typename TYPE>
+__device__ __host__ HZ_INLINE void forward_promise(PROGRAM program, ENUM pe, ReturnAdr<TYPE> adr) {
+    using Set = typename PROGRAM::OpSet;
+    if constexpr (I < OpCount<Set>::value) {
+        using Op = typename OpOf<I, Set>::type;
+        if constexpr (std::is_same<typename Promise<Op>::Return, TYPE>::value) {
+            if (pe.disc == I) {
+                Promise<Op> p = pe.data.template cast<Op>();
+                p.ret = adr;
+                program.async_call(p);
+                return;
+            }
+        }
+        forward_promise<I + 1>(program, pe, adr);
+    }
+}
+
+}  // namespace detail
+
 template <typename PROGRAM, typename TYPE>
 __device__ __host__ HZ_INLINE void resolve_value(PROGRAM program, ReturnAdr<TYPE> adr, const TYPE& value) {
     if (adr.dest != nullptr) *adr.dest = value;
@@ -199,12 +224,9 @@ __device__ __host__ HZ_INLINE void resolve(PROGRAM program, ReturnAdr<TYPE> adr,
         if (result.form == RESULT::VALUE) {
             resolve_value(program, adr, result.data.value);
         } else if (result.form == RESULT::PROMISE) {
-            // Return's constructor admitted only promises of operations that
-            // return TYPE, and a promise's return address is its first member
-            // in every such case, so it is set without dispatching on disc.
-            typename RESULT::EnumType pe = result.data.promise;
-            __builtin_memcpy((void*)&pe.data, &adr, sizeof(adr));
-            program.async_call(pe);
+            // The returned promise (of an operation returning TYPE, as Return's
+            // constructor checked) takes over this operation's return address.
+            detail::forward_promise<0>(program, result.data.promise, adr);
         } else {
             ReturnAdr<TYPE> src = result.data.future;
             Promise<ReturnOp> forward((TaggedSemaphore*)adr.base, (void*)adr.dest, (void*)src.dest, sizeof(TYPE));

@@ -319,8 +319,6 @@ union PromiseUnion<OpUnion<>> {
     template <typename PROGRAM>
     __device__ __host__ void loose_eval(PROGRAM, OpDisc) {}
     HZ_HD void dyn_copy_as(OpDisc, PromiseUnion<OpUnion<>>&) {}
-    template <typename TYPE>
-    HZ_HD void dyn_set_return(OpDisc, ReturnAdr<TYPE>) {}
     PromiseUnion() = default;
 };
 
@@ -396,16 +394,6 @@ union PromiseUnion<OpUnion<HEAD, TAIL...>> {
         }
     }
 
-    // Sets the return address of the member with discriminant `disc`, which
-    // must be a promise of an operation returning TYPE.
-    template <typename TYPE>
-    HZ_HD void dyn_set_return(OpDisc disc, ReturnAdr<TYPE> adr) {
-        if (disc == 0) {
-            if constexpr (std::is_same<typename Head::Return, TYPE>::value) head_form.ret = adr;
-        } else {
-            if constexpr (sizeof...(TAIL) > 0) tail_form.dyn_set_return(disc - 1, adr);
-        }
-    }
 };
 
 // A promise union tagged with its discriminant.
