@@ -84,3 +84,35 @@ def test_trivial_starts():
 def test_leaf_count_must_be_a_power_of_two():
     with pytest.raises(ValueError):
         O.futures_program(np.arange(1, 4, dtype=np.uint64), readers=0)
+
+
+def test_library_checks_futures_arguments_before_any_device_work():
+    # hz_futures_run validates its arguments before touching the GPU, so the
+    # checks are exercised here without one.
+    import ctypes
+
+    import harmonize_amd as hz
+
+    lib = hz.lib()
+    total = ctypes.c_uint64(0)
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every case fails its checks first
+
+    def run(kind, leaves, readers, starts=fake, echo=fake, fwd=fake, out=ctypes.byref(total)):
+        rc = lib.hz_futures_run(kind, starts, leaves, readers, echo, fwd, out, None, None)
+        msg = lib.hz_last_error()
+        return rc, (msg.decode() if msg else "")
+
+    rc, msg = run(hz.ASYNC, 3, 0)
+    assert rc != 0 and "power of two" in msg
+    rc, msg = run(hz.ASYNC, 0, 0)
+    assert rc != 0 and "power of two" in msg
+    rc, msg = run(hz.ASYNC, 1 << 27, 0)
+    assert rc != 0 and "power of two" in msg
+    rc, msg = run(hz.ASYNC, 4, (1 << 24) + 1)
+    assert rc != 0 and "readers" in msg
+    rc, msg = run(hz.ASYNC, 4, 2, echo=None)
+    assert rc != 0 and "null" in msg
+    rc, msg = run(hz.EVENT, 4, 2)
+    assert rc != 0 and "EventProgram" in msg
+    rc, msg = run(7, 4, 0)
+    assert rc != 0 and "kind" in msg
