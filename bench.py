@@ -233,12 +233,12 @@ def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -
     load, for arrival-order and largest-first admission."""
     out = {
         "workload": f"{jobs} jobs, Collatz / photon walk / BFS with p = 1/3 each, sizes log-uniform in "
-                    f"[2^10, 2^22] items, per-job seeds; 4 slots (concurrent streams); the same mix at every load",
+                    f"[2^10, 2^22] items, per-job seeds; 3 slots (concurrent streams); the same mix at every load",
     }
     mix = serving_mix(jobs, 1)
     unit = np.random.default_rng(2).exponential(1.0, size=len(mix))  # Poisson gaps at rate 1
     for name, policy in (("fifo", hz.SERVE_FIFO), ("largest_first", hz.SERVE_LARGEST_FIRST)):
-        srv = hz.Server(pool_bytes=48 << 30, slots=4, policy=policy)
+        srv = hz.Server(pool_bytes=48 << 30, slots=3, policy=policy)
         try:
             warm = serving_mix(12, 99)
             srv.run_trace(warm, np.zeros(len(warm)))

@@ -467,7 +467,8 @@ typedef struct hz_server hz_server;
 typedef struct hz_server_config {
     uint64_t pool_bytes;  /* device memory for job state (0 = 8 GiB) */
     uint64_t arena_links; /* per slot and async program (0 = 2^16) */
-    uint32_t slots;       /* concurrent jobs, 1..8 (0 = 4, the hardware queues of a process) */
+    uint32_t slots;       /* concurrent jobs, 1..8 (0 = 3: a process has 4 hardware queues and the
+                             default stream uses one) */
     uint32_t max_pending; /* submissions beyond this many waiting jobs fail (0 = no bound) */
     uint32_t keep_outputs; /* 1: a finished job keeps its state until hz_server_release */
     uint32_t policy;       /* admission order: HZ_SERVE_FIFO (0) or HZ_SERVE_LARGEST_FIRST */
