@@ -95,3 +95,22 @@ def test_python_binding_signatures_cover_header():
     # Every name the Python binding types must exist in the header.
     declared = set(declared_symbols())
     assert set(hz._SIGNATURES) <= declared
+
+
+def test_missing_library_fails_loudly():
+    # No CPU fallback: without the built runtime every entry point raises.
+    import subprocess
+    import sys
+
+    code = (
+        "import harmonize_amd as hz, pathlib\n"
+        "hz.LIB_PATH = pathlib.Path('/nonexistent/libharmonize_amd.so')\n"
+        "hz._lib = None\n"
+        "try:\n"
+        "    hz.collatz_make_starts(1, 40, 16)\n"
+        "except hz.HarmonizeError as e:\n"
+        "    print('raised', 'not built' in str(e))\n"
+    )
+    proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                          cwd=str(ROOT))
+    assert proc.stdout.strip() == "raised True", proc.stdout + proc.stderr
