@@ -316,6 +316,13 @@ def _leg_bfs(kind):
     return run
 
 
+def _leg_futures(kind):
+    def run(hz):
+        starts = hz.DeviceArray.from_numpy(np.arange(1, FUTURES_LEAVES + 1, dtype=np.uint64))
+        return hz.futures_run(kind, starts, FUTURES_READERS if kind == "async" else 0)[3]
+    return run
+
+
 LEGS = {
     "bfs_async": _leg_bfs("async"),
     "bfs_event": _leg_bfs("event"),
@@ -324,6 +331,8 @@ LEGS = {
     "photon_async": _leg_photon("async"),
     "photon_event": _leg_photon("event"),
     "copy": _leg_copy,
+    "futures_async": _leg_futures("async"),
+    "futures_event": _leg_futures("event"),
 }
 
 
