@@ -170,7 +170,7 @@ def bfs_comparison(hz, repeats: int = 3) -> dict:
                 runs.append(batch.run(kind, max_launches=1 << 16, **cfg))
             return min(runs, key=lambda r: r["elapsed_ms"]), batch.levels()
 
-        tuned = hz.tune_bfs(batch, rounds=1, reps=1)
+        tuned = hz.tune_bfs(batch, rounds=2, reps=2)
         cfg = {k: v for k, v in tuned["best"].items() if v and k != "max_launches"}
         cfg["flags"] = hz.RUN_GRAPH
         a, lev_a = best("async", **cfg)
