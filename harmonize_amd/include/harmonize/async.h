@@ -152,6 +152,7 @@ class AsyncProgram {
     static_assert(GROUPS_PER_BLOCK >= 1 && GROUPS_PER_BLOCK <= 16, "GROUPS_PER_BLOCK must be in [1, 16]");
     static const size_t WORK_GROUP_SIZE = 64 * GROUPS_PER_BLOCK;
     static constexpr bool SHARE_WORK = hz::spec::SHARE_WORKOr<PROGRAM_SPEC, 0>::value != 0;
+    static constexpr bool OLDEST_FIRST = hz::spec::OLDEST_FIRSTOr<PROGRAM_SPEC, 0>::value != 0;
     // Groups live for the whole launch (specifications may keep group-level
     // partial results, e.g. LDS tallies, and flush them in finalize).
     static constexpr bool PERSISTENT = true;
@@ -216,6 +217,8 @@ class AsyncProgram {
         uint32_t words[LINKS][WORDS][64];
         uint32_t link_op[LINKS];
         uint32_t link_count[LINKS];
+        uint32_t link_seq[OLDEST_FIRST ? LINKS : 1];  // fill order of ready links (OLDEST_FIRST)
+        uint32_t seq;
         uint32_t open[OPS > 0 ? OPS : 1];
         uint32_t free_mask;
         uint32_t ready_mask;
@@ -508,6 +511,20 @@ class AsyncProgram {
     // -------------------------------------------------------------- the stash
     __device__ static uint32_t lowest(uint32_t mask) { return (uint32_t)__builtin_ctz(mask); }
 
+    // The ready link of `mask` (non-empty) that filled first (OLDEST_FIRST).
+    __device__ HZ_INLINE uint32_t oldest(uint32_t mask) {
+        uint32_t best_link = lowest(mask), best = 0xFFFFFFFFu;
+        for (uint32_t m = mask; m != 0u; m &= m - 1u) {
+            uint32_t i = lowest(m);
+            uint32_t q = hz::wave::first(_grp_ctx.link_seq[i]);
+            if (q < best) {
+                best = q;
+                best_link = i;
+            }
+        }
+        return best_link;
+    }
+
     // Moves stash link L to the arena and queues it. Cooperative over the
     // active lanes (all 64 in the scheduler loop, a subset inside an async
     // call). Returns false if the arena is exhausted.
@@ -608,6 +625,7 @@ class AsyncProgram {
                 g.link_count[L] = meta >> 16;
                 g.free_mask = fm & ~(1u << L);
                 g.ready_mask |= (1u << L);
+                if constexpr (OLDEST_FIRST) g.link_seq[L] = g.seq++;
                 g.fills += 1;
                 arena_free_lane(id);
                 if (g.holding) {
@@ -659,6 +677,11 @@ class AsyncProgram {
         uint32_t rmask = hz::wave::first(g.ready_mask);
         g.link_count[L] = 64u;
         g.ready_mask = rmask | (1u << L);
+        if constexpr (OLDEST_FIRST) {
+            uint32_t q = hz::wave::first(g.seq);
+            g.link_seq[L] = q;
+            g.seq = q + 1u;
+        }
         g.open[D] = hz::LINK_NONE;
         hz::wave::sync();
         uint32_t rest = n - room;
@@ -894,7 +917,7 @@ class AsyncProgram {
         {
             HZ_PHASE(PHASE_PICK);
             if (rm != 0) {
-                L = lowest(rm);
+                L = OLDEST_FIRST ? oldest(rm) : lowest(rm);
             } else {
                 uint32_t c = 0;
                 L = fullest_open(c);
@@ -959,6 +982,7 @@ class AsyncProgram {
         if (hz::wave::lane() == 0) {
             g.free_mask = ALL_LINKS;
             g.ready_mask = 0;
+            g.seq = 0;
             for (uint32_t d = 0; d < OPS; ++d) g.open[d] = hz::LINK_NONE;
             g.spare_count = 0;
             g.holding = 0;

@@ -52,6 +52,11 @@ HZ_CONST_OR(GROUPS_PER_BLOCK)
 // Nonzero: a group that holds several links hands some to groups that have
 // none (for work that grows from few seeds). Not a reference constant.
 HZ_CONST_OR(SHARE_WORK)
+// Nonzero: a group runs its ready links oldest first (the order in which
+// they filled), rather than by stash position. Work whose later promises can
+// supersede earlier ones (a BFS level lowered again) wastes less. Not a
+// reference constant.
+HZ_CONST_OR(OLDEST_FIRST)
 #undef HZ_CONST_OR
 
 // The operation set a program type executes: the specification's OpSet, plus

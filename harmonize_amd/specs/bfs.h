@@ -112,6 +112,11 @@ struct BfsSpecT {
     static const size_t GROUPS_PER_BLOCK = GROUPS;
     // A search grows from one vertex per graph: hand links to idle groups.
     static const size_t SHARE_WORK = 1;
+    // Run the links that filled first first, closer to level order: fewer
+    // vertices are visited again at a lower level (77 M -> 69 M promises at
+    // the default configuration, 57.6 -> 50.7 ms; the photon walk and
+    // Collatz are faster without it).
+    static const size_t OLDEST_FIRST = 1;
 
     template <typename PROGRAM>
     __device__ static void initialize(PROGRAM) {}
