@@ -152,6 +152,41 @@ def futures_comparison(hz, repeats: int = 3) -> dict:
     }
 
 
+BARRIER_DEP, BARRIER_NON_DEP = 1 << 21, 1 << 21
+
+
+def barrier_demo(hz, repeats: int = 3) -> dict:
+    """The reference's remapping-barrier demo (examples/cpp/barrier_demo/demo.cpp)
+    at scale: 2^21 dependent steps, each awaiting barrier[id % barriers] before
+    it runs again, and 2^21 independent steps. Checked as the demo checks it
+    (counter = 2 * dependents + independents, demo.cpp:214), plus every held
+    step run exactly once and none before its barrier's release. Measured with
+    1,024 barriers (2,048 awaits each) and with 64 (32,768 awaits each), where
+    the awaits of one barrier contend on its partial-link word."""
+
+    def best(barriers, reps):
+        runs = []
+        for _ in range(reps):
+            counter, hits, violations, st = hz.barrier_run(BARRIER_DEP, BARRIER_NON_DEP, barriers, arena=1 << 17,
+                                                           max_launches=1000)
+            if counter != 2 * BARRIER_DEP + BARRIER_NON_DEP or violations != 0 or not bool((hits == 1).all()):
+                raise RuntimeError(f"barrier demo ({barriers} barriers): counter {counter}, violations {violations}")
+            runs.append(st)
+        return min(runs, key=lambda r: r["elapsed_ms"])
+
+    out = {"workload": "remapping-barrier demo: 2^21 dependent steps awaiting barrier[id % barriers], "
+                       "2^21 independent steps (AsyncProgram; EventProgram cannot hold promises on a barrier)"}
+    for barriers, reps in ((1024, repeats), (64, 1)):
+        r = best(barriers, reps)
+        out[f"barriers_{barriers}"] = {
+            "ms": r["elapsed_ms"],
+            "held_promises_per_s": r["promises"] / (r["elapsed_ms"] * 1e-3),
+            "steps_per_s": (2 * BARRIER_DEP + BARRIER_NON_DEP) / (r["elapsed_ms"] * 1e-3),
+            "held_links": r["links_run"],
+        }
+    return out
+
+
 BFS_SEED = 0x5EED
 BFS_GRAPHS = 64
 
@@ -316,6 +351,14 @@ def _leg_bfs(kind):
     return run
 
 
+def _leg_barrier(hz):
+    counter, hits, violations, st = hz.barrier_run(BARRIER_DEP, BARRIER_NON_DEP, 1024, arena=1 << 17,
+                                                   max_launches=1000)
+    if counter != 2 * BARRIER_DEP + BARRIER_NON_DEP or violations != 0:
+        raise RuntimeError("barrier demo: wrong result")
+    return st
+
+
 def _leg_futures(kind):
     def run(hz):
         starts = hz.DeviceArray.from_numpy(np.arange(1, FUTURES_LEAVES + 1, dtype=np.uint64))
@@ -324,6 +367,7 @@ def _leg_futures(kind):
 
 
 LEGS = {
+    "barrier_async": _leg_barrier,
     "bfs_async": _leg_bfs("async"),
     "bfs_event": _leg_bfs("event"),
     "collatz_async": _leg_collatz("async"),
@@ -348,6 +392,7 @@ def main() -> int:
     ap.add_argument("--no-bfs", action="store_true", help="skip the batched BFS Async vs Event comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving load sweep")
     ap.add_argument("--no-futures", action="store_true", help="skip the value-returning futures tree")
+    ap.add_argument("--no-barrier", action="store_true", help="skip the remapping-barrier demo")
     ap.add_argument("--leg", choices=sorted(LEGS), help="run one leg once (for counter collection) and exit")
     args = ap.parse_args()
 
@@ -447,6 +492,8 @@ def main() -> int:
 
     if not args.no_futures:
         out["futures"] = futures_comparison(hz)
+    if not args.no_barrier:
+        out["barrier"] = barrier_demo(hz)
 
     if not args.no_serve:
         out["serving"] = serving_sweep(hz)

@@ -13,8 +13,8 @@ def test_bench_imports_and_defines_legs():
     sys.path.insert(0, str(ROOT))
     import bench
 
-    assert set(bench.LEGS) == {"bfs_async", "bfs_event", "collatz_async", "collatz_event", "photon_async",
-                               "photon_event", "copy", "futures_async", "futures_event"}
+    assert set(bench.LEGS) == {"barrier_async", "bfs_async", "bfs_event", "collatz_async", "collatz_event",
+                               "photon_async", "photon_event", "copy", "futures_async", "futures_event"}
     assert bench.measured_traffic_bytes() > 0
     mix = bench.serving_mix(1000, 1)
     kinds = {k for k, _, _ in mix}
