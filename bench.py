@@ -8,9 +8,14 @@ starts (init + exec launches until complete); the starts stay resident in HBM.
 Prints one JSON line: promises resolved per second of AsyncProgram, with the
 EventProgram and one-thread-per-start comparators, the roofline of the
 dominant kernel and the CPU baseline (the oracle's host executors on a
-bounded sample of the same workload).
+bounded sample of the same workload). Further sections, each checked against
+its known result before it is reported: the photon walk and the batched BFS
+(Async vs Event), the futures tree, the remapping-barrier demo and the
+serving load sweep (p50/p99 latency at 20-100% of peak).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-event]
+                    [--no-photon] [--no-bfs] [--no-futures] [--no-barrier] [--no-serve]
+    python bench.py --leg <name>   # one leg once, for counter collection
 """
 
 from __future__ import annotations
