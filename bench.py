@@ -9,10 +9,14 @@ Prints one JSON line: promises resolved per second of AsyncProgram, with the
 EventProgram and one-thread-per-start comparators, the roofline of the
 dominant kernel and the CPU baseline (the oracle's host executors on a
 bounded sample of the same workload). Further sections: the photon walk and
-the batched BFS (Async vs Event), the futures tree and the remapping-barrier
-demo, each of which fails if its result is wrong, and the serving load sweep
-(p50/p99 latency at 20-100% of peak). The Collatz and serving outputs are
-checked against the oracle by the GPU tests, not here.
+the batched BFS (Async vs Event), the futures tree, the remapping-barrier demo
+and the serving load sweep (p50/p99 latency at 20-100% of peak).
+
+Consistency checks made here (a failure raises): the Collatz step counts of
+AsyncProgram, EventProgram and the one-thread-per-start kernel are identical;
+photon event totals and BFS levels agree between the two programs; the futures
+total and the barrier demo's count match their closed forms. The GPU tests
+check every workload, serving included, against the oracle.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-event]
                     [--no-photon] [--no-bfs] [--no-futures] [--no-barrier] [--no-serve]
@@ -429,6 +433,7 @@ def main() -> int:
         runs.append(hz.collatz_run("async", steps_out, starts, COUNT, **cfg))
     hz.check(hz.lib().hz_device_synchronize(), "sync")
     wall = time.perf_counter() - t0
+    steps_async = steps_out.numpy()  # compared below with the EventProgram and one-thread-per-start runs
 
     promises = runs[-1]["promises"]
     total_promises = sum(r["promises"] for r in runs)
@@ -481,6 +486,8 @@ def main() -> int:
 
     if not args.no_event:
         ev = hz.collatz_run("event", steps_out, starts, COUNT, max_launches=1 << 20)
+        if not np.array_equal(steps_out.numpy(), steps_async):
+            raise RuntimeError("collatz: AsyncProgram and EventProgram step counts differ")
         out["event"] = {
             "value": ev["promises"] / (ev["elapsed_ms"] * 1e-3),
             "unit": "promises/s",
@@ -505,6 +512,8 @@ def main() -> int:
         out["serving"] = serving_sweep(hz)
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
+    if not np.array_equal(steps_out.numpy(), steps_async):
+        raise RuntimeError("collatz: AsyncProgram and one-thread-per-start step counts differ")
     out["plain_kernel"] = {
         "ms": pl["elapsed_ms"],
         "steps_per_s": promises / (pl["elapsed_ms"] * 1e-3),
