@@ -15,6 +15,9 @@
 // Levels converge to the exact BFS distances (each final level is attained
 // by a visit that relaxed along a shortest path); unreached vertices keep
 // BFS_UNREACHED. The number of visits depends on the order, the levels do not.
+// On the GPU the edge ranges left after splitting are relaxed by the whole
+// wavefront together (bfs_relax_wave), so R-MAT's skewed degrees do not leave
+// lanes idle; the host executors relax each range in its own lane.
 #pragma once
 
 #include <cstdint>
@@ -42,6 +45,83 @@ struct BfsState {
 struct BfsVisit;
 struct BfsEdges;
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// One lane's edge range, as the wavefront shares it in LDS (32 bytes).
+struct BfsRange {
+    const int32_t* col;  // first column of the range
+    int32_t* level;      // the job's level array
+    unsigned int start;  // exclusive prefix of the wavefront's range lengths
+    unsigned int job;
+    int next;            // level written to the range's targets
+    unsigned int pad;
+};
+
+// Relaxes the edge ranges of all active lanes together: edge e of the
+// concatenated ranges goes to lane (e mod active count), so every lane has
+// work until the wavefront's edges run out, whatever the degree skew. Each
+// active lane publishes its range at its rank among the active lanes (rows
+// [0, n) of its wavefront's LDS block are exactly the live ones), and the
+// prefix of range lengths comes from ballots over their bit slices, so
+// inactive lanes are never read.
+template <typename PROGRAM>
+__device__ inline void bfs_relax_wave(PROGRAM& prog, unsigned int job, const BfsJob& j, int begin, int end,
+                                      int next) {
+    constexpr unsigned int WAVES = (unsigned int)(PROGRAM::WORK_GROUP_SIZE / 64);
+    __shared__ BfsRange ranges[WAVES][64];
+    BfsRange* row = ranges[hz_block_thread() / 64u];
+    const uint64_t live = hz::wave::active();
+    const unsigned int n = hz::wave::popc(live);
+    const unsigned int me = hz::wave::rank(live);
+    const unsigned int len = (unsigned int)(end - begin);
+    unsigned int start = 0u, total = 0u;
+    for (unsigned int b = 0; b < 32u && hz::wave::ballot((len >> b) != 0u) != 0ull; ++b) {
+        const uint64_t m = hz::wave::ballot(((len >> b) & 1u) != 0u);
+        start += hz::wave::rank(m) << b;
+        total += hz::wave::popc(m) << b;
+    }
+    row[me] = BfsRange{j.col + begin, j.level, start, job, next, 0u};
+    hz::wave::sync();
+    // Four edges per lane per round keep four memory chains in flight.
+    for (unsigned int base = 0; base < total; base += 4u * n) {
+        int w[4], seen[4], before[4];
+        const BfsRange* owner[4];
+        bool has[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned int e = base + (unsigned int)k * n + me;
+            has[k] = e < total;
+            owner[k] = row;
+            w[k] = 0;
+            if (has[k]) {
+                // The last range starting at or before e holds it: ranges
+                // of length 0 share their successor's start.
+                unsigned int lo = 0u, hi = n - 1u;
+                while (lo < hi) {
+                    const unsigned int mid = (lo + hi + 1u) >> 1;
+                    if (row[mid].start <= e) lo = mid; else hi = mid - 1u;
+                }
+                owner[k] = row + lo;
+                w[k] = owner[k]->col[e - owner[k]->start];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) seen[k] = has[k] ? hz_load(&owner[k]->level[w[k]]) : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int nx = owner[k]->next;
+            before[k] = has[k] && seen[k] > nx ? atomicMin(&owner[k]->level[w[k]], nx) : nx;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (has[k] && before[k] > owner[k]->next) {
+                prog.template async<BfsVisit>(owner[k]->job, w[k], owner[k]->next);
+            }
+        }
+    }
+    hz::wave::sync();
+}
+#endif
+
 template <typename PROGRAM>
 __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& j, int begin, int end, int next) {
     const int split = (int)prog.device.split;
@@ -53,6 +133,9 @@ __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& 
         }
         end = begin + piece;
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+    bfs_relax_wave(prog, job, j, begin, end, next);
+#else
     // Four edges at a time: their column reads, level reads and atomics are
     // independent, so a lane keeps four memory chains in flight.
     int i = begin;
@@ -76,6 +159,7 @@ __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& 
             if (before > next) prog.template async<BfsVisit>(job, w, next);
         }
     }
+#endif
 }
 
 struct BfsVisit {
