@@ -18,6 +18,7 @@ using BfsEvent = EventProgram<BfsSpecT<1>>;
 
 static_assert(sizeof(BfsJob) == 32, "BfsJob layout");
 static_assert(sizeof(hz_bfs_state) == sizeof(BfsState), "C and C++ BFS states must match");
+static_assert(HZ_BFS_MAX_SPLIT == BFS_MAX_SPLIT, "C and C++ split limits must match");
 
 HZ_DEFINE_PROGRAM_ABI(bfs_event, BfsEvent, BfsState)
 HZ_DEFINE_PROGRAM_ABI(bfs_async, BfsAsync1, BfsState)
@@ -457,10 +458,14 @@ extern "C" int hz_bfs_batch_layout(const hz_bfs_batch* batch, uint32_t* vertices
 
 extern "C" int hz_bfs_batch_run(hz_bfs_batch* batch, int kind, const hz_run_config* config, hz_run_stats* stats) {
     return guarded([&] {
-        check_batch(batch);
         hz_run_config c{};
         if (config) c = *config;
         if (stats) std::memset(stats, 0, sizeof(*stats));
+        if (c.spec_param > BFS_MAX_SPLIT) {
+            throw std::invalid_argument("hz_bfs_batch_run: the split threshold (spec_param) must be in [1, 2^24] "
+                                        "(0 = default 64)");
+        }
+        check_batch(batch);
         if (kind == HZ_EVENT) {
             if (c.flags & HZ_RUN_GRAPH) throw std::invalid_argument("hz_bfs_batch_run: graph launch is for HZ_ASYNC");
             run_event(*batch, c, stats);

@@ -73,6 +73,12 @@ extern "C" int hz_instance_create(int kind, int spec, uint64_t size, const void*
                                         std::to_string(e->state_bytes) + " bytes)");
         }
         if (size == 0) throw std::invalid_argument("hz_instance_create: size must be positive");
+        if (spec == HZ_SPEC_BFS) {
+            const hz_bfs_state* bs = static_cast<const hz_bfs_state*>(device_state);
+            if (bs->split < 1u || bs->split > HZ_BFS_MAX_SPLIT) {
+                throw std::invalid_argument("hz_instance_create: hz_bfs_state.split must be in [1, 2^24]");
+            }
+        }
         auto inst = new hz_instance();
         inst->entry = e;
         try {

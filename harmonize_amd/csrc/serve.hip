@@ -45,6 +45,14 @@ using SBfs = EventProgram<BfsSpecT<1>>;
 
 constexpr uint32_t COLLATZ_BITS = 32;      // serving Collatz starts are uniform in [1, 2^32)
 constexpr uint32_t PHOTON_REPLICAS = 64;   // tally copies per photon job
+
+// Depth bins of a serving photon job: the workload default, the same value
+// the launch zeroes and the job's tally layout reserves.
+inline uint32_t photon_job_bins() {
+    hz_photon_params p;
+    hz_photon_defaults(&p);
+    return p.bins;
+}
 constexpr uint32_t BFS_EDGE_FACTOR = 16;
 constexpr uint64_t ALIGN = 256;
 constexpr uint32_t CHUNK_MIN_ITEMS = 1u << 18;  // smallest chunk a job is cut into
@@ -150,7 +158,7 @@ Layout layout_for(const hz_job& j, uint32_t chunks) {
             break;
         case HZ_JOB_PHOTON:
             L.a = take(4 * n);  // events
-            L.b = take((uint64_t)chunks * PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8);  // tally per chunk
+            L.b = take((uint64_t)chunks * PHOTON_REPLICAS * (photon_job_bins() + HZ_PHOTON_EXTRA_SLOTS) * 8);  // tally per chunk
             break;
         case HZ_JOB_BFS: {
             uint64_t E = n * BFS_EDGE_FACTOR;
@@ -671,7 +679,7 @@ extern "C" int hz_server_output(hz_server* server, uint64_t id, uint32_t which, 
                 break;
             case HZ_JOB_PHOTON:
                 if (which == 0) sub = j.layout.a, want = 4 * n;          // events
-                else if (which == 1) sub = j.layout.b, want = (uint64_t)j.chunks * PHOTON_REPLICAS * (1000 + HZ_PHOTON_EXTRA_SLOTS) * 8;
+                else if (which == 1) sub = j.layout.b, want = (uint64_t)j.chunks * PHOTON_REPLICAS * (photon_job_bins() + HZ_PHOTON_EXTRA_SLOTS) * 8;
                 break;
             case HZ_JOB_BFS:
                 if (which == 0) sub = j.layout.c, want = 4 * n;          // levels

@@ -24,6 +24,11 @@
 
 constexpr int32_t BFS_UNREACHED = 0x7FFFFFFF;
 constexpr uint32_t BFS_DEFAULT_SPLIT = 64u;
+// Valid split thresholds are [1, BFS_MAX_SPLIT]: a threshold of 0 would cut a
+// one-edge range forever, and the wavefront's prefix of range lengths
+// (64 lanes x split) must fit in 32 bits. The device clamps to this range;
+// the entry points reject values outside it.
+constexpr uint32_t BFS_MAX_SPLIT = 1u << 24;
 constexpr int BFS_FANOUT = 16;
 
 // One graph of the batch (32 bytes).
@@ -124,7 +129,10 @@ __device__ inline void bfs_relax_wave(PROGRAM& prog, unsigned int job, const Bfs
 
 template <typename PROGRAM>
 __device__ inline void bfs_relax(PROGRAM& prog, unsigned int job, const BfsJob& j, int begin, int end, int next) {
-    const int split = (int)prog.device.split;
+    unsigned int split_limit = prog.device.split;
+    if (split_limit < 1u) split_limit = 1u;
+    if (split_limit > BFS_MAX_SPLIT) split_limit = BFS_MAX_SPLIT;
+    const int split = (int)split_limit;
     while (end - begin > split) {
         int piece = (end - begin + BFS_FANOUT - 1) / BFS_FANOUT;
         if (piece < split) piece = split;

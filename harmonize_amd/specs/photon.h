@@ -99,6 +99,54 @@ __host__ __device__ inline float float_of_bits(unsigned int b) {
 // 2^-k for 0 <= k < 126, built from its bit pattern.
 __host__ __device__ inline float pow2_neg(unsigned int k) { return float_of_bits((127u - k) << 23); }
 
+// 2^k for 0 <= k <= 127, built from its bit pattern (exact; a shift of a
+// 32-bit integer would be undefined from k = 32 on).
+__host__ __device__ inline float pow2(unsigned int k) { return float_of_bits((127u + k) << 23); }
+
+// Outcome of the weight window at one collision.
+enum WindowFate : unsigned int {
+    WINDOW_KEEP = 0,      // inside the window: one copy at the same weight
+    WINDOW_SURVIVE = 1,   // roulette won: one copy at the target weight
+    WINDOW_DIE = 2,       // roulette lost
+    WINDOW_SPLIT = 3,     // `copies` copies of weight w / copies
+};
+
+struct WindowResult {
+    unsigned int fate;
+    unsigned int copies;
+    float w;  // weight of each copy (the weight lost, for WINDOW_DIE)
+};
+
+// Weight window at a collision in region `region` for a photon of weight w
+// (after implicit capture). With target = 2^-region and r = w / target:
+//  - r < roulette_below: roulette with u = (rbits >> 8) / 2^24 uniform in
+//    [0, 1): survive with probability r at the target weight (so the expected
+//    weight is w), else die;
+//  - r > split_above: c = the largest power of two <= min(r, max_split)
+//    copies of weight w / c (exact, c is a power of two);
+//  - otherwise one copy of weight w.
+__host__ __device__ inline WindowResult weight_window(const PhotonState& st, float w, unsigned int region,
+                                                      unsigned int rbits) {
+    WindowResult out{WINDOW_KEEP, 1u, w};
+    float ratio = w * pow2(region);  // w / 2^-region, exact
+    if (ratio < st.roulette_below) {
+        float u2 = (float)(rbits >> 8) * (1.0f / 16777216.0f);  // [0, 1)
+        if (u2 >= ratio) {
+            out.fate = WINDOW_DIE;
+            return out;
+        }
+        out.fate = WINDOW_SURVIVE;
+        out.w = pow2_neg(region);
+    } else if (ratio > st.split_above) {
+        unsigned int k = 0;
+        while ((2u << k) <= st.max_split && (float)(2u << k) <= ratio) ++k;
+        out.fate = k > 0 ? WINDOW_SPLIT : WINDOW_KEEP;
+        out.copies = 1u << k;
+        out.w = w * pow2_neg(k);  // exact
+    }
+    return out;
+}
+
 // Integer hash "lowbias32" (C. Wellons, 2018) of (lineage, event, copy).
 __host__ __device__ inline unsigned int child_lineage(unsigned int lineage, unsigned int event, unsigned int copy) {
     unsigned int h = lineage ^ (event * 0x9E3779B9u) ^ ((copy + 1u) * 0x85EBCA6Bu);
@@ -213,24 +261,17 @@ struct PhotonStep {
 
         unsigned int region = (unsigned int)(nz * st.region_scale);
         if (region >= st.regions) region = st.regions - 1u;
-        float ratio = w * (float)(1u << region);  // w / 2^-region, exact
-        unsigned int copies = 1u;
-        if (ratio < st.roulette_below) {
-            float u2 = (float)(r.v[1] >> 8) * (1.0f / 16777216.0f);  // [0, 1)
-            if (u2 >= ratio) {
-                tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_LOST, (double)w);
-                atomicAdd(&st.events[p.id], (unsigned int)p.pending);
-                return;
-            }
-            float target = photon_detail::pow2_neg(region);
-            tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_GAINED, (double)target - (double)w);
-            w = target;
-        } else if (ratio > st.split_above) {
-            unsigned int k = 0;
-            while ((2u << k) <= st.max_split && (float)(2u << k) <= ratio) ++k;
-            copies = 1u << k;
-            w = w * photon_detail::pow2_neg(k);  // exact
+        photon_detail::WindowResult win = photon_detail::weight_window(st, w, region, r.v[1]);
+        if (win.fate == photon_detail::WINDOW_DIE) {
+            tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_LOST, (double)w);
+            atomicAdd(&st.events[p.id], (unsigned int)p.pending);
+            return;
         }
+        if (win.fate == photon_detail::WINDOW_SURVIVE) {
+            tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_GAINED, (double)win.w - (double)w);
+        }
+        const unsigned int copies = win.copies;
+        w = win.w;
 
         if (p.event + 1u >= st.max_events) {
             tally_add<PROGRAM>(st, st.bins + PHOTON_CAPPED, (double)w * (double)copies);

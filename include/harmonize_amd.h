@@ -106,6 +106,7 @@ typedef struct hz_photon_state {
  * edge offsets, col[edges] of local vertex ids) with an int32 level array;
  * unreached vertices keep HZ_BFS_UNREACHED. */
 #define HZ_BFS_UNREACHED 0x7FFFFFFF
+#define HZ_BFS_MAX_SPLIT (1u << 24) /* largest edge-range split threshold */
 typedef struct hz_bfs_job {
     const int32_t* row_ptr;
     const int32_t* col;
@@ -117,7 +118,10 @@ typedef struct hz_bfs_state {
     const hz_bfs_job* jobs;
     void* job_iter; /* {uint32 next, uint32 limit}: jobs still to start */
     uint32_t job_count;
-    uint32_t split; /* edge ranges longer than this are halved (0 is invalid) */
+    uint32_t split; /* edge ranges longer than this are cut into up to 16
+                       pieces of at least `split` edges; valid range
+                       [1, 2^24] (hz_instance_create rejects others, the
+                       device clamps to it) */
 } hz_bfs_state;
 
 typedef struct hz_barrier_state {
@@ -285,7 +289,8 @@ typedef struct hz_run_config {
     uint32_t flags;        /* HZ_RUN_GRAPH: launch the pass through a captured
                               hipGraph (BFS batches, HZ_ASYNC) */
     uint32_t spec_param;   /* program-specific knob (BFS: longest edge range a
-                              promise relaxes before splitting it; 0 = 64) */
+                              promise relaxes before splitting it, at most
+                              2^24; 0 = 64) */
     uint32_t stash;        /* async: LDS links per group where the program
                               offers variants (BFS: 8 or 16; 0 = 16) */
 } hz_run_config;

@@ -114,3 +114,28 @@ def test_missing_library_fails_loudly():
     proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
                           cwd=str(ROOT))
     assert proc.stdout.strip() == "raised True", proc.stdout + proc.stderr
+
+
+class _BfsState(ctypes.Structure):
+    _fields_ = [("jobs", ctypes.c_void_p), ("job_iter", ctypes.c_void_p), ("job_count", ctypes.c_uint32),
+                ("split", ctypes.c_uint32)]
+
+
+@pytest.mark.parametrize("split", [0, (1 << 24) + 1, 0xFFFFFFFF])
+def test_bfs_split_out_of_range_is_refused_before_any_launch(split):
+    # A split threshold of 0 would cut a one-edge range forever and one above
+    # 2^24 would overflow the wavefront's 32-bit prefix of range lengths; both
+    # entry points refuse them before touching the device (no GPU here).
+    lib = hz.lib()
+    state = _BfsState(None, None, 1, split)
+    h = ctypes.c_void_p()
+    rc = lib.hz_instance_create(hz.KINDS["async"], hz.SPEC_BFS, 1024, ctypes.byref(state), ctypes.sizeof(state),
+                                ctypes.byref(h))
+    assert rc != 0 and h.value is None
+    assert "split must be in [1, 2^24]" in hz.last_error()
+    if split > (1 << 24):
+        cfg = hz.RunConfig()
+        cfg.spec_param = split
+        rc = lib.hz_bfs_batch_run(None, hz.KINDS["async"], ctypes.byref(cfg), None)
+        assert rc != 0
+        assert "split threshold" in hz.last_error()
