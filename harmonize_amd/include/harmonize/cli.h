@@ -1,14 +1,19 @@
-// Command-line flags for host mains (reference util/cli.h:314-497):
-// `cli::ArgSet args(argc, argv); unsigned n = args["limit"] | 1024u;`
-// A flag is `-name value`; a flag given without a value reads as true when
-// converted to bool; `| default` supplies the value of an absent flag.
+// Command-line utilities for host mains (reference util/cli.h):
+//  - flags (util/cli.h:314-497):
+//    `cli::ArgSet args(argc, argv); unsigned n = args["limit"] | 1024u;`
+//    A flag is `-name value`; a flag given without a value reads as true
+//    when converted to bool; `| default` supplies the value of an absent flag.
+//  - text plots (util/cli.h:13-71, :93): `cli_graph(data, size, shape, style)`
+//    draws a series as a bar plot on the terminal with block-element tiles.
 // Not on the execution path.
 #pragma once
 
 #include <cerrno>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
 
 namespace util {
@@ -130,6 +135,98 @@ struct ArgSet {
 
     ArgQuery operator[](char const* flag_str) const { return ArgQuery(flag_str, get_flag_str(flag_str)); }
 };
+
+// A tile set for cli_graph. One character cell shows `cols` sub-columns side
+// by side, each filled from the bottom to a level in [0, rows]; the cell for
+// levels (l_0, ..., l_{cols-1}) is tiles[sum_k l_k * (rows+1)^(cols-1-k)].
+struct GraphStyle {
+    int rows;
+    int cols;
+    const char** tiles;
+    GraphStyle(unsigned int r, unsigned int c, const char** t) : rows((int)r), cols((int)c), tiles(t) {}
+};
+
+// Plot size in character cells and the value window it shows.
+struct GraphShape {
+    int width;
+    int height;
+    float x_min;
+    float x_max;
+    float y_min;
+    float y_max;
+};
+
+// Eighth blocks: one sub-column, eight levels per cell.
+static const char* EighthBlockTiles[] = {" ", "\u2581", "\u2582", "\u2583", "\u2584",
+                                         "\u2585", "\u2586", "\u2587", "\u2588"};
+// Quadrant blocks: two sub-columns (left, right) of two levels each.
+static const char* QuadrantTiles[] = {
+    " ",      "\u2597", "\u2590",  // left empty:  right 0, 1, 2
+    "\u2596", "\u2584", "\u259F",  // left lower half
+    "\u258C", "\u2599", "\u2588",  // left full
+};
+static const GraphStyle BlockFill = GraphStyle(8, 1, EighthBlockTiles);
+static const GraphStyle Block2x2Fill = GraphStyle(2, 2, QuadrantTiles);
+
+// Draws data[0, size) as `shape.height` text rows of `shape.width` cells.
+// Each sub-column shows the mean of the samples that fall into it (an empty
+// one repeats its left neighbour); the y axis is labelled per row and the x
+// axis every fourth cell, both from the shape's window.
+inline void cli_graph(float* data, int size, GraphShape shape, GraphStyle style) {
+    if (data == nullptr || size <= 0 || shape.width <= 0 || shape.height <= 0 || style.rows <= 0 ||
+        style.cols <= 0) {
+        return;
+    }
+    const int subs = shape.width * style.cols;
+    float* means = new float[subs];
+    float carry = data[0];
+    for (int s = 0; s < subs; ++s) {
+        long long lo = (long long)s * size / subs;
+        long long hi = (long long)(s + 1) * size / subs;
+        if (hi > lo) {
+            double acc = 0.0;
+            for (long long i = lo; i < hi; ++i) acc += data[i];
+            carry = (float)(acc / (double)(hi - lo));
+        }
+        means[s] = carry;
+    }
+    const float span = shape.y_max - shape.y_min;
+    const float band = span / (float)shape.height;
+    for (int row = 0; row < shape.height; ++row) {
+        const float floor_val = shape.y_min + band * (float)(shape.height - 1 - row);
+        std::printf("%11.4g |", (double)floor_val);
+        for (int c = 0; c < shape.width; ++c) {
+            int index = 0;
+            for (int k = 0; k < style.cols; ++k) {
+                float v = means[c * style.cols + k];
+                int level = 0;
+                if (band > 0.0f) {
+                    float rel = (v - floor_val) / band * (float)style.rows;
+                    level = rel <= 0.0f ? 0 : (rel >= (float)style.rows ? style.rows : (int)rel);
+                } else if (v >= floor_val) {
+                    level = style.rows;
+                }
+                index = index * (style.rows + 1) + level;
+            }
+            std::fputs(style.tiles[index], stdout);
+        }
+        std::printf("\n");
+    }
+    std::printf("%11s +", "");
+    for (int c = 0; c < shape.width; ++c) std::fputs(c % 4 == 0 ? "+" : "-", stdout);
+    std::printf("\n%13s", "");
+    std::string axis;
+    for (int c = 0; c < shape.width; c += 4) {
+        char label[32];
+        float x = shape.x_min + (shape.x_max - shape.x_min) * (float)c / (float)shape.width;
+        std::snprintf(label, sizeof(label), "%-4.3g", (double)x);
+        std::string cell(label);
+        cell.resize(4, ' ');
+        axis += cell.substr(0, 4);
+    }
+    std::printf("%s\n", axis.c_str());
+    delete[] means;
+}
 
 }  // namespace cli
 }  // namespace util

@@ -1,12 +1,17 @@
-"""The reference's own example programs (examples/cpp/collaz) built unchanged
-against this runtime's harmonize.h by oracle/build_ref.sh: on the GPU they
-run to completion and their built-in checker (collaz_hrm.cpp:157-169) finds
-no bad output, for the EventProgram as written and for the AsyncProgram
-refactor. Skipped where the binaries were not built (no reference sources)."""
+"""The reference's own example programs built unchanged against this
+runtime's harmonize.h by oracle/build_ref.sh:
+ - examples/cpp/collaz: on the GPU they run to completion and their built-in
+   checker (collaz_hrm.cpp:157-169) finds no bad output, for the
+   EventProgram as written and for the AsyncProgram refactor;
+ - examples/cpp/example_neutron/program.cpp, which instantiates both program
+   types: its EventProgram run (default) and AsyncProgram run (-async) print
+   the same census (-csv).
+Skipped where the binaries were not built (no reference sources)."""
 
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 REF = Path(__file__).resolve().parents[1] / "oracle" / "_ref"
@@ -37,3 +42,33 @@ def test_reference_plain_collatz_kernel_runs():
     assert proc.returncode == 0, proc.stderr[-2000:]
     assert "Bad output" not in proc.stdout
     assert "Runtime:" in proc.stdout
+
+
+def neutron_census(*extra):
+    # A subcritical slab (k = mult * fx / (fx + cx) = 0.4): 2^14 source
+    # neutrons, census at t = 1 in 32 bins of width 0.25 over [-4, 4).
+    proc = run(
+        "neut",
+        "-wg_count", 1024, "-num", 1 << 14, "-pool", 1 << 22,
+        "-fx", 0.2, "-cx", 0.8, "-sx", 1.0, "-mult", 2,
+        "-time", 1.0, "-size", 4.0, "-res", 0.25, "-csv", *extra,
+    )
+    assert proc.returncode == 0, proc.stderr[-2000:]
+    rows = [line for line in proc.stdout.splitlines() if line.strip()]
+    assert len(rows) == 1, proc.stdout[-2000:]
+    return np.array([float(x) for x in rows[0].split(",")])
+
+
+@pytest.mark.gpu
+def test_reference_neutron_program_event_and_async_agree():
+    # Every neutron's history is a function of its id alone (its seed and its
+    # children's seeds derive from its own state, logic.cpp:40-56, 179-248),
+    # so both program types run the same histories and their census tallies
+    # differ only by the order of the float atomic adds (logic.cpp:222):
+    # relative 1e-4 covers that reordering of 2^14-sample float32 sums, and
+    # 2e-6 absolute the CSV's six printed decimals (program.cpp:357-366).
+    event = neutron_census()
+    async_ = neutron_census("-async")
+    assert event.shape == (32,)
+    assert event.sum() > 0.1  # neutrons reach the census
+    np.testing.assert_allclose(async_, event, rtol=1e-4, atol=2e-6)
