@@ -104,14 +104,21 @@ def test_from_csr_rejects_bad_input(hz):
         hz.BfsBatch.from_csr([3], [2], [0, 2, 1, 2], [0, 1])  # row_ptr decreases
 
 
-def test_rmat_batch_64_graphs(hz):
-    """The 64-graph batch, 2^14..2^22 vertices, 16 edges per vertex: levels of
-    both program types bit-exact against the textbook BFS on the same CSR; the
-    CSR of the eight smallest graphs bit-exact against the host build."""
+@pytest.fixture(scope="module")
+def full_batch(hz):
     sizes = hz.bfs_batch_sizes(SEED, 64, 14.0, 22.0)
     b = hz.BfsBatch.rmat(SEED, sizes)
     row_ptr, col = b.csr()
     want = O.bfs_levels(b.vertices, b.edges, row_ptr, col)
+    yield b, sizes, row_ptr, col, want
+    b.close()
+
+
+def test_rmat_batch_64_graphs(hz, full_batch):
+    """The 64-graph batch, 2^14..2^22 vertices, 16 edges per vertex: levels of
+    both program types bit-exact against the textbook BFS on the same CSR; the
+    CSR of the eight smallest graphs bit-exact against the host build."""
+    b, sizes, row_ptr, col, want = full_batch
     for kind in ("async", "event"):
         b.run(kind, max_launches=1 << 16)
         assert np.array_equal(b.levels(), want), kind
@@ -120,7 +127,42 @@ def test_rmat_batch_64_graphs(hz):
         r_dev = row_ptr[b.row_base[g]: b.row_base[g] + sizes[g] + 1]
         c_dev = col[b.edge_base[g]: b.edge_base[g] + b.edges[g]]
         assert np.array_equal(r_dev, r_host) and np.array_equal(c_dev, c_host), f"graph {g}"
-    b.close()
+
+
+@pytest.mark.parametrize(
+    "cfg",
+    [
+        # a configuration the benchmark's tuner chose for this batch
+        {"groups": 2048, "arena": 466741, "queues": 16, "coalesce": 32, "block_groups": 2, "spec_param": 128,
+         "stash": 8},
+        {"groups": 2048, "arena": 466741, "queues": 16, "coalesce": 32, "block_groups": 4, "spec_param": 64,
+         "stash": 16},
+    ],
+)
+def test_rmat_batch_64_graphs_captured_at_tuned_shapes(hz, full_batch, cfg):
+    """The 64-graph batch through the captured hipGraph pass at the shapes the
+    benchmark times, replayed twice with an eager pass in between."""
+    b, _, _, _, want = full_batch
+    for replay in range(2):
+        b.run("async", flags=hz.RUN_GRAPH, max_launches=1 << 16, **cfg)
+        assert np.array_equal(b.levels(), want), f"replay {replay}"
+        b.run("event")
+        assert np.array_equal(b.levels(), want)
+    b.release()
+
+
+def test_rmat_batch_64_graphs_at_the_tuners_choice(hz, full_batch):
+    """The tuner's own pick on the 64-graph batch (one round of single trials),
+    captured and replayed, against the textbook BFS."""
+    b, _, _, _, want = full_batch
+    res = hz.tune_bfs(b, rounds=1, reps=1)
+    best = {k: v for k, v in res["best"].items() if v}
+    assert best.get("flags", 0) & hz.RUN_GRAPH
+    best.setdefault("max_launches", 1 << 16)
+    for replay in range(2):
+        b.run("async", **best)
+        assert np.array_equal(b.levels(), want), f"replay {replay} of {best}"
+    b.release()
 
 
 def test_tuner_picks_a_working_configuration(hz, small_batch):

@@ -59,10 +59,20 @@ def c1_full():
     return n, dev, ref, promises
 
 
-@pytest.mark.parametrize("kind", ["plain", "async", "event"])
-def test_c1_full_scale_random_starts(c1_full, kind):
+@pytest.mark.parametrize(
+    "kind,cfg",
+    [
+        ("plain", {}),
+        ("async", {}),
+        ("event", {}),
+        # the configuration the benchmark times: an arena of 2^24 promises
+        # (2^18 links) behind 16 link queues
+        ("async", {"arena": 1 << 18, "queues": 16}),
+    ],
+)
+def test_c1_full_scale_random_starts(c1_full, kind, cfg):
     n, dev, ref, promises = c1_full
-    got, st = run(kind, n, starts=dev, max_launches=100000)
+    got, st = run(kind, n, starts=dev, max_launches=100000, **cfg)
     assert np.array_equal(got, ref)
     assert int((got == hz.COLLATZ_OVERFLOW).sum()) == 3
     if kind != "plain":
