@@ -101,6 +101,9 @@ _SIGNATURES = {
         [ctypes.c_int, _VP, ctypes.c_uint32, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
     ),
     "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+    "hz_collatz_run_noalloc": (
+        ctypes.c_int, [_VP, ctypes.c_uint32, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
+    ),
 }
 
 PROGRAM_SUFFIXES = ("collatz_event", "collatz_async", "photon_event", "photon_async", "bfs_event", "bfs_async",
@@ -241,6 +244,21 @@ def collatz_run(kind, steps: DeviceArray, starts: DeviceArray | None = None, cou
         lib().hz_collatz_run(k, starts.ptr if starts is not None else None, n, steps.ptr, ctypes.byref(config),
                              ctypes.byref(st)),
         "hz_collatz_run",
+    )
+    return st.as_dict()
+
+
+def collatz_run_noalloc(steps: DeviceArray, starts: DeviceArray | None = None, count: int | None = None,
+                        **cfg) -> dict:
+    """Runs the Collatz AsyncProgram with the runtime's buffers in the
+    library's fixed device array (hz_collatz_run_noalloc); returns its statistics."""
+    n = steps.count if count is None else int(count)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(
+        lib().hz_collatz_run_noalloc(starts.ptr if starts is not None else None, n, steps.ptr, ctypes.byref(config),
+                                     ctypes.byref(st)),
+        "hz_collatz_run_noalloc",
     )
     return st.as_dict()
 

@@ -355,26 +355,7 @@ class AsyncProgram {
         }
 
         // True once every promise has been evaluated and make_work is exhausted.
-        bool complete() {
-            read_status();
-            if (host_status.flags & ARENA_EXHAUSTED_FLAG) {
-                throw std::runtime_error(
-                    "AsyncProgram: the link arena was exhausted and promises were lost; "
-                    "construct the Instance with a larger arena");
-            }
-            if (host_status.flags & USAGE_FLAG) {
-                throw std::runtime_error(
-                    "AsyncProgram: a Return forwarded a future that cannot hold promises "
-                    "(a unit future that was not ready)");
-            }
-            if (host_status.flags & QUEUE_STALL_FLAG) {
-                char detail[64];
-                std::snprintf(detail, sizeof(detail), "%016llx", (unsigned long long)host_status.stall_detail);
-                throw std::runtime_error(std::string("AsyncProgram: a bounded wait gave up (") +
-                                         hz::stall_name(host_status.stall_site) + ", detail 0x" + detail + ")");
-            }
-            return (host_status.flags & COMPLETION_FLAG) != 0;
-        }
+        bool complete() { return status_complete(read_status()); }
 
         void clear_flags() {
             unsigned int zero = 0;
@@ -383,6 +364,29 @@ class AsyncProgram {
 
         unsigned long long int promises() { return read_status().promises; }
     };
+
+    // Reads a status copied back from the device: raises if the run lost
+    // promises or a bounded wait gave up, and returns true once every promise
+    // has been evaluated and make_work is exhausted.
+    static bool status_complete(const hz::AsyncStatus& st) {
+        if (st.flags & ARENA_EXHAUSTED_FLAG) {
+            throw std::runtime_error(
+                "AsyncProgram: the link arena was exhausted and promises were lost; "
+                "construct the Instance with a larger arena");
+        }
+        if (st.flags & USAGE_FLAG) {
+            throw std::runtime_error(
+                "AsyncProgram: a Return forwarded a future that cannot hold promises "
+                "(a unit future that was not ready)");
+        }
+        if (st.flags & QUEUE_STALL_FLAG) {
+            char detail[64];
+            std::snprintf(detail, sizeof(detail), "%016llx", (unsigned long long)st.stall_detail);
+            throw std::runtime_error(std::string("AsyncProgram: a bounded wait gave up (") +
+                                     hz::stall_name(st.stall_site) + ", detail 0x" + detail + ")");
+        }
+        return (st.flags & COMPLETION_FLAG) != 0;
+    }
 
     DeviceContext& _dev_ctx;
     GroupContext& _grp_ctx;

@@ -320,6 +320,25 @@ typedef struct hz_run_stats {
 int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count, uint32_t* d_steps,
                    const hz_run_config* config, hz_run_stats* stats);
 
+/* Allocation-free variant of hz_collatz_run(HZ_ASYNC, ...). It has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code that sets up HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It runs the same kernels, but the runtime's own
+ * buffers (claim iterator, status, link queues, link arena) live in one fixed
+ * device array of at most HZ_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; the call allocates no device memory. The caller passes the
+ * inputs and outputs as for hz_collatz_run. Limits: config->arena at most
+ * HZ_NOALLOC_MAX_LINKS links (0 = that maximum), config->queues at most
+ * HZ_NOALLOC_MAX_QUEUES (0 = 8); larger values are refused with
+ * HZ_ERR_ARGUMENT before the device is touched. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (several host threads)
+ * race and must be serialized by the caller. elapsed_ms is host wall time. */
+#define HZ_NOALLOC_MAX_LINKS 16384u
+#define HZ_NOALLOC_MAX_QUEUES 16u
+#define HZ_NOALLOC_SCRATCH_BYTES (18u << 20)
+int hz_collatz_run_noalloc(const uint64_t* d_starts, uint32_t count, uint32_t* d_steps,
+                           const hz_run_config* config, hz_run_stats* stats);
+
 /* Fills d_out[count] with starts drawn uniformly from [1, 2^bits) by
  * Philox4x32-10 keyed by seed: start i = (((x1 << 32) | x0) mod (2^bits - 1)) + 1
  * where (x0, x1, x2, x3) = Philox4x32-10(counter = {i, 0, 0, 0},

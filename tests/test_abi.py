@@ -139,3 +139,16 @@ def test_bfs_split_out_of_range_is_refused_before_any_launch(split):
         rc = lib.hz_bfs_batch_run(None, hz.KINDS["async"], ctypes.byref(cfg), None)
         assert rc != 0
         assert "split threshold" in hz.last_error()
+
+
+@pytest.mark.parametrize("field,value", [("arena", 16385), ("queues", 17)])
+def test_noalloc_refuses_sizes_beyond_its_static_array_before_any_launch(field, value):
+    # The allocation-free entry point's buffers live in a fixed device array:
+    # configurations that would not fit are refused before the device is
+    # touched (no GPU here).
+    cfg = hz.RunConfig()
+    setattr(cfg, field, value)
+    st = hz.RunStats()
+    rc = hz.lib().hz_collatz_run_noalloc(None, 0, None, ctypes.byref(cfg), ctypes.byref(st))
+    assert rc == -1
+    assert "static maximum" in hz.last_error()
