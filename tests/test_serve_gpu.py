@@ -50,10 +50,12 @@ def check_against_oracle(hz, srv, i, kind, size, seed):
 
 
 @pytest.mark.parametrize("kind,size", [("collatz", 1), ("collatz", 5000), ("collatz", (1 << 20) + 7),
+                                       ("collatz", 1 << 22),
                                        ("photon", 1), ("photon", 20000), ("photon", (1 << 19) + 3),
-                                       ("bfs", 1), ("bfs", 3000)])
+                                       ("photon", 1 << 22),
+                                       ("bfs", 1), ("bfs", 3000), ("bfs", 1 << 22)])
 def test_job_matches_oracle(hz, kind, size):
-    srv = hz.Server(pool_bytes=1 << 30, slots=2, keep_outputs=True)
+    srv = hz.Server(pool_bytes=4 << 30, slots=2, keep_outputs=True)
     try:
         i = srv.submit(kind, size, 0xA11CE + size)
         srv.drain()
@@ -102,6 +104,35 @@ def test_alone_equals_mix(hz):
         assert overlaps > 0
         assert len({r["slot"] for r in res}) == 4
         check_against_oracle(hz, srv, res[0]["id"], *jobs[0])
+    finally:
+        srv.close()
+
+
+def test_alone_equals_poisson_mix_full_size_range(hz):
+    """The serving mix's own law: sizes log-uniform over [2^10, 2^22] items,
+    Poisson arrivals (here at ~400 jobs/s, so jobs queue and overlap); every
+    result identical to the same job run alone."""
+    jobs = make_mix(24, 11, lo=10, hi=22)
+    rng = np.random.default_rng(12)
+    arrivals = np.cumsum(rng.exponential(2.5, size=len(jobs)))
+    srv = hz.Server(pool_bytes=8 << 30, slots=3, keep_outputs=True)
+    try:
+        alone = []
+        for kind, size, seed in jobs:
+            i = srv.submit(kind, size, seed)
+            srv.drain()
+            alone.append(outputs(srv, i, kind))
+            srv.release(i)
+        res = srv.run_trace(jobs, arrivals)
+        assert all(r["status"] == hz.JOB_DONE for r in res)
+        for (kind, size, seed), r, a in zip(jobs, res, alone):
+            got = outputs(srv, r["id"], kind)
+            assert np.array_equal(got[0], a[0]), (kind, size, seed)
+            if kind == "photon":
+                assert_close(got[1], a[1])
+        big = max(range(len(jobs)), key=lambda k: jobs[k][1])
+        assert jobs[big][1] > 1 << 20
+        check_against_oracle(hz, srv, res[big]["id"], *jobs[big])
     finally:
         srv.close()
 
