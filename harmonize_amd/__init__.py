@@ -45,6 +45,7 @@ class RunConfig(ctypes.Structure):
 
 
 RUN_GRAPH = 1
+RUN_AUDIT = 2
 
 
 class RunStats(ctypes.Structure):
@@ -55,16 +56,17 @@ class RunStats(ctypes.Structure):
         ("fills", ctypes.c_uint64),
         ("make_work_calls", ctypes.c_uint64),
         ("groups", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("audits", ctypes.c_uint32),
         ("elapsed_ms", ctypes.c_double),
         ("phase_cycles", ctypes.c_uint64 * 8),
         ("links_run", ctypes.c_uint64),
+        ("arena_links", ctypes.c_uint64),
     ]
 
     PHASES = ("cycle", "spill", "pull", "make_work", "pick", "read", "eval", "idle")
 
     def as_dict(self) -> dict:
-        out = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("reserved", "phase_cycles")}
+        out = {name: getattr(self, name) for name, _ in self._fields_ if name != "phase_cycles"}
         if any(self.phase_cycles):
             out["phase_cycles"] = dict(zip(self.PHASES, [int(x) for x in self.phase_cycles]))
         return out
@@ -101,6 +103,12 @@ _SIGNATURES = {
         [ctypes.c_int, _VP, ctypes.c_uint32, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
     ),
     "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+    "hz_audit_selftest": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "hz_tree_run": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP,
+         ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
+    ),
     "hz_collatz_run_noalloc": (
         ctypes.c_int, [_VP, ctypes.c_uint32, _VP, ctypes.POINTER(RunConfig), ctypes.POINTER(RunStats)],
     ),
@@ -324,6 +332,21 @@ def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
     check(lib().hz_barrier_run(dep_count, non_dep_count, barrier_count, counter.ptr, hits.ptr, violations.ptr,
                                ctypes.byref(config), ctypes.byref(st)), "hz_barrier_run")
     return int(counter.numpy()[0]), hits.numpy()[:dep_count], int(violations.numpy()[0]), st.as_dict()
+
+
+TREE_FLAT, TREE_DEEPEST, TREE_SHALLOWEST = 0, 1, 2
+
+
+def tree_run(kind, order: int, roots: int, fanout: int, height: int, **cfg):
+    """Runs the depth-level tree program; returns (leaves[roots], violations, stats)."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    leaves = DeviceArray(max(1, roots), np.uint32)
+    violations = DeviceArray(1, np.uint32)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(lib().hz_tree_run(k, order, roots, fanout, height, leaves.ptr, violations.ptr, ctypes.byref(config),
+                            ctypes.byref(st)), "hz_tree_run")
+    return leaves.numpy()[:roots], int(violations.numpy()[0]), st.as_dict()
 
 
 FUTURES_UNRESOLVED = (1 << 64) - 1  # a reader that ran early, or a future left unresolved

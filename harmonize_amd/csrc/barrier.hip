@@ -55,6 +55,7 @@ extern "C" int hz_barrier_run(uint32_t dep_count, uint32_t non_dep_count, uint32
         BarrierAsync::Instance inst(arena, st, cfg.queues, cfg.coalesce ? cfg.coalesce : 64u);
         init<BarrierAsync>(inst, groups);
         check(hipDeviceSynchronize(), "init");
-        hz::capi::run_async<BarrierAsync>(inst, groups, cycles, cfg.max_launches, stats);
+        hz::capi::run_async<BarrierAsync>(inst, groups, cycles, cfg.max_launches, stats,
+                                          (cfg.flags & HZ_RUN_AUDIT) != 0, true);
     });
 }

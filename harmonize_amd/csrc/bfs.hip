@@ -79,7 +79,8 @@ struct StreamTimer {
 
 struct Runner {
     virtual ~Runner() = default;
-    virtual void run(hz_run_stats* stats) = 0;
+    // audit: check the queues and arena on the host after every launch.
+    virtual void run(hz_run_stats* stats, bool audit) = 0;
 };
 
 }  // namespace
@@ -204,7 +205,7 @@ struct AsyncRunner : Runner {
         exec<P>(inst, groups, cycles, b.stream);
     }
 
-    void run(hz_run_stats* stats) override {
+    void run(hz_run_stats* stats, bool audit) override {
         StreamTimer t(b.stream);
         t.start();
         if (graph_exec) {
@@ -215,6 +216,17 @@ struct AsyncRunner : Runner {
         check(hipGetLastError(), "bfs launch");
         check(hipStreamSynchronize(b.stream), "bfs exec");
         uint64_t launches = 1, last = ~0ull;
+        uint32_t audits = 0;
+        auto audit_now = [&] {
+            if (!audit) return;
+            const bool done = inst.complete();  // raises on a runtime error first
+            hz::AsyncCensus c = inst.audit(false);
+            audits++;
+            if (done && c.queued_links != 0) {
+                throw std::runtime_error("AsyncProgram audit: complete with links still queued");
+            }
+        };
+        audit_now();
         // A pass that ends early (cycle budget) resumes with plain launches.
         while (!inst.complete()) {
             if (max_launches && launches >= max_launches) {
@@ -227,6 +239,7 @@ struct AsyncRunner : Runner {
             check(hipGetLastError(), "bfs relaunch");
             check(hipStreamSynchronize(b.stream), "bfs exec");
             launches++;
+            audit_now();
         }
         double ms = t.stop();
         if (stats) {
@@ -241,6 +254,8 @@ struct AsyncRunner : Runner {
             stats->elapsed_ms = ms;
             for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
             stats->links_run = s.links_run;
+            stats->audits = audits;
+            stats->arena_links = s.claim < inst.arena_size ? s.claim : inst.arena_size;
         }
     }
 };
@@ -473,7 +488,7 @@ extern "C" int hz_bfs_batch_run(hz_bfs_batch* batch, int kind, const hz_run_conf
             auto key = runner_key(c);
             auto it = batch->runners.find(key);
             if (it == batch->runners.end()) it = batch->runners.emplace(key, make_async_runner(*batch, c)).first;
-            it->second->run(stats);
+            it->second->run(stats, (c.flags & HZ_RUN_AUDIT) != 0);
         } else {
             throw std::invalid_argument("hz_bfs_batch_run: kind must be HZ_EVENT or HZ_ASYNC");
         }

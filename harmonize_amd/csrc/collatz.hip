@@ -93,7 +93,8 @@ extern "C" int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count
             CollatzAsync::Instance inst(arena, ds, cfg.queues, cfg.coalesce ? cfg.coalesce : 64u);
             init<CollatzAsync>(inst, groups);
             check(hipDeviceSynchronize(), "init");
-            hz::capi::run_async<CollatzAsync>(inst, groups, cycles, cfg.max_launches, stats);
+            hz::capi::run_async<CollatzAsync>(inst, groups, cycles, cfg.max_launches, stats,
+                                              (cfg.flags & HZ_RUN_AUDIT) != 0);
         } else if (kind == HZ_EVENT) {
             uint32_t groups = cfg.groups ? cfg.groups : default_groups_event();
             uint64_t io = cfg.arena;

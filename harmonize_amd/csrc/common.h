@@ -93,13 +93,16 @@ struct EventTimer {
 };
 
 // Runs an AsyncProgram instance to completion: exec launches until complete().
+// With `audit`, the instance's queues and arena are audited after every
+// launch (Instance::audit) and the number of audits is reported.
 template <typename P>
 void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uint32_t max_launches,
-               hz_run_stats* stats) {
+               hz_run_stats* stats, bool audit = false, bool barriers = false) {
     EventTimer t;
     t.start();
     uint64_t launches = 0;
     uint64_t last_progress = ~0ull;
+    uint32_t audits = 0;
     do {
         if (max_launches && launches >= max_launches) {
             throw std::runtime_error("AsyncProgram: launch bound reached before completion");
@@ -108,7 +111,15 @@ void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uin
         check(hipGetLastError(), "exec launch");
         check(hipDeviceSynchronize(), "exec");
         launches++;
-        if (inst.complete()) break;
+        const bool done = inst.complete();  // raises on a runtime error first
+        if (audit) {
+            hz::AsyncCensus c = inst.audit(barriers);
+            audits++;
+            if (done && (c.queued_links != 0 || c.held_links != 0)) {
+                throw std::runtime_error("AsyncProgram audit: complete with links still queued or held");
+            }
+        }
+        if (done) break;
         // A launch that evaluated no promise signals lost accounting rather
         // than unfinished work: report it instead of looping.
         hz::AsyncStatus s = inst.read_status();
@@ -130,6 +141,8 @@ void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uin
         stats->elapsed_ms = ms;
         for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
         stats->links_run = s.links_run;
+        stats->audits = audits;
+        stats->arena_links = s.claim < inst.arena_size ? s.claim : inst.arena_size;
     }
 }
 

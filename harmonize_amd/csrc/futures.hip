@@ -86,7 +86,8 @@ extern "C" int hz_futures_run(int kind, const uint64_t* d_starts, uint32_t leave
             FuturesAsync::Instance inst(arena, st, cfg.queues, cfg.coalesce ? cfg.coalesce : 64u);
             init<FuturesAsync>(inst, groups);
             check(hipDeviceSynchronize(), "init");
-            hz::capi::run_async<FuturesAsync>(inst, groups, cycles, cfg.max_launches, stats);
+            hz::capi::run_async<FuturesAsync>(inst, groups, cycles, cfg.max_launches, stats,
+                                              (cfg.flags & HZ_RUN_AUDIT) != 0, true);
         } else {
             uint32_t groups = cfg.groups ? cfg.groups : (uint32_t)hz::capi::resident_groups_of<FuturesEvent>();
             uint64_t io = cfg.arena;

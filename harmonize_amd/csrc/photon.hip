@@ -81,7 +81,8 @@ extern "C" int hz_photon_run(int kind, const hz_photon_params* params, double* d
             PhotonAsync::Instance inst(arena, st, cfg.queues, cfg.coalesce ? cfg.coalesce : 64u);
             init<PhotonAsync>(inst, groups);
             check(hipDeviceSynchronize(), "init");
-            hz::capi::run_async<PhotonAsync>(inst, groups, cycles, cfg.max_launches, stats);
+            hz::capi::run_async<PhotonAsync>(inst, groups, cycles, cfg.max_launches, stats,
+                                             (cfg.flags & HZ_RUN_AUDIT) != 0);
         } else if (kind == HZ_EVENT) {
             uint32_t groups = cfg.groups ? cfg.groups : (uint32_t)hz::capi::compute_units() * 32u;
             uint64_t io = cfg.arena;

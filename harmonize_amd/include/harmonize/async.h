@@ -30,7 +30,9 @@
 #pragma once
 
 #include <cstdio>
+#include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "host.h"
 #include "iter.h"
@@ -71,6 +73,10 @@ struct AsyncStatus {
     unsigned long long int launches;
     unsigned long long int idle_polls;
     unsigned long long int links_run;  // links evaluated (promises / links_run = mean lanes busy)
+    // Promises written into links: by async calls into a stash link and by
+    // barrier awaits into a barrier link. Every promise written is evaluated
+    // exactly once, so appended = promises + those still queued or held.
+    unsigned long long int appended;
     // Shader-clock cycles per scheduler phase, summed over groups; filled only
     // when the runtime is built with HZ_PHASE_TIMING (the counterpart of the
     // reference's HRM_TIME phase timers, preamble/mod.h:59-67).
@@ -90,6 +96,113 @@ enum AsyncPhase : unsigned int {
 };
 
 constexpr unsigned int LINK_NONE = 0xFFFFFFFFu;
+
+// What a host-side audit of an AsyncProgram instance found between launches
+// (AsyncProgram::Instance::audit).
+struct AsyncCensus {
+    unsigned long long claimed;          // arena links ever handed out
+    unsigned long long queued_links;     // links parked in the work queues
+    unsigned long long queued_promises;  // promises in those links
+    unsigned long long free_links;       // links in the free ring
+    unsigned long long held_links;       // links held by barriers (neither queued nor free)
+    unsigned long long held_promises;    // promises written into held links
+    unsigned long long evaluated;        // promises evaluated so far
+    unsigned long long appended;         // promises written into links so far
+};
+
+// A host copy of an AsyncProgram instance's status, rings and link metadata,
+// taken between launches.
+struct AsyncSnapshot {
+    AsyncStatus status{};
+    std::vector<LinkQueue> queues;    // the work queues, then the free ring
+    std::vector<unsigned int> slots;  // queues.size() rings of `capacity` slots
+    unsigned int capacity = 0;        // power of two
+    std::vector<unsigned int> meta;   // per claimed link: op | level << 8 | count << 16
+    std::vector<unsigned int> aux;    // per claimed link: next, (target << 16) | filled
+    unsigned int arena_links = 0;
+    unsigned int ops = 0;
+    unsigned int levels = 1;
+};
+
+// Checks a snapshot: every group has left (checkout, makers and hungry are
+// zero), each ring's count equals tail - head and exactly its slots in
+// [head, tail) hold link ids, no link id is queued twice or both queued and
+// free, units equals the queued links, every queued link names a valid
+// operation and holds 1..64 promises, and every promise written into a link
+// is accounted for: appended = evaluated + queued + held. Claimed links that
+// are neither queued nor free are held by barriers (allowed only if
+// `allow_held`). Throws std::runtime_error naming the first violation.
+inline AsyncCensus audit_snapshot(const AsyncSnapshot& s, bool allow_held) {
+    auto fail = [](const std::string& what) { throw std::runtime_error("AsyncProgram audit: " + what); };
+    const AsyncStatus& st = s.status;
+    if (st.checkout != 0 || st.makers != 0 || st.hungry != 0) {
+        fail("groups still checked in (checkout " + std::to_string(st.checkout) + ", makers " +
+             std::to_string(st.makers) + ", hungry " + std::to_string(st.hungry) + ")");
+    }
+    const unsigned int cap = s.capacity;
+    if (cap == 0 || (cap & (cap - 1u)) != 0 || s.slots.size() != (size_t)s.queues.size() * cap) {
+        fail("malformed snapshot");
+    }
+    // The bump counter may overshoot the arena (claims that lost the race).
+    const unsigned int claimed = st.claim < s.arena_links ? st.claim : s.arena_links;
+    if (s.meta.size() < claimed || s.aux.size() < 2 * (size_t)claimed) fail("malformed snapshot");
+    std::vector<unsigned char> seen(claimed, 0);  // 1 = queued for work, 2 = free
+    AsyncCensus c{};
+    c.claimed = claimed;
+    c.evaluated = st.promises;
+    c.appended = st.appended;
+    const size_t nq = s.queues.size();
+    for (size_t q = 0; q < nq; ++q) {
+        const bool is_free = q + 1 == nq;
+        const std::string name = is_free ? std::string("free ring") : "queue " + std::to_string(q);
+        const LinkQueue& Q = s.queues[q];
+        const unsigned int span = Q.tail - Q.head;
+        if (Q.count < 0 || (unsigned int)Q.count != span || span > cap) {
+            fail(name + ": count " + std::to_string(Q.count) + " but tail - head = " + std::to_string(span));
+        }
+        const unsigned int* ring = s.slots.data() + q * cap;
+        for (unsigned int k = 0; k < cap; ++k) {
+            const unsigned int pos = Q.head + k;
+            const unsigned int id = ring[pos & (cap - 1u)];
+            if (k >= span) {
+                if (id != LINK_NONE) fail(name + ": occupied slot outside [head, tail)");
+                continue;
+            }
+            if (id == LINK_NONE || id >= claimed) fail(name + ": slot " + std::to_string(pos) + " holds no valid link");
+            if (seen[id]) fail("link " + std::to_string(id) + " appears twice in the rings");
+            seen[id] = is_free ? 2 : 1;
+            if (is_free) {
+                c.free_links++;
+                continue;
+            }
+            const unsigned int op = s.meta[id] & 0xFFu, level = (s.meta[id] >> 8) & 0xFFu, count = s.meta[id] >> 16;
+            if (op >= s.ops) fail("queued link " + std::to_string(id) + " names operation " + std::to_string(op));
+            if (level >= s.levels) fail("queued link " + std::to_string(id) + " names level " + std::to_string(level));
+            if (count < 1 || count > 64) {
+                fail("queued link " + std::to_string(id) + " holds " + std::to_string(count) + " promises");
+            }
+            c.queued_links++;
+            c.queued_promises += count;
+        }
+    }
+    for (unsigned int id = 0; id < claimed; ++id) {
+        if (seen[id]) continue;
+        c.held_links++;
+        const unsigned int filled = s.aux[2u * id + 1u] & 0xFFFFu;
+        if (filled > 64) fail("held link " + std::to_string(id) + " counts " + std::to_string(filled) + " promises");
+        c.held_promises += filled;
+    }
+    if (!allow_held && c.held_links != 0) fail(std::to_string(c.held_links) + " arena links are neither queued nor free");
+    if (st.units < 0 || (unsigned long long)st.units != c.queued_links) {
+        fail("units " + std::to_string(st.units) + " but " + std::to_string(c.queued_links) + " queued links");
+    }
+    if (c.appended != c.evaluated + c.queued_promises + c.held_promises) {
+        fail("promises unaccounted for: appended " + std::to_string(c.appended) + ", evaluated " +
+             std::to_string(c.evaluated) + ", queued " + std::to_string(c.queued_promises) + ", held " +
+             std::to_string(c.held_promises));
+    }
+    return c;
+}
 
 // Bounded waits of the runtime; the first one to give up is recorded.
 enum AsyncStall : unsigned int {
@@ -157,10 +270,23 @@ class AsyncProgram {
     // partial results, e.g. LDS tallies, and flush them in finalize).
     static constexpr bool PERSISTENT = true;
 
-    static_assert(STACK_SIZE == 0, "only a flat work stack is supported (as in the reference)");
+    // Depth levels of the work stack (reference WorkStack, mem/work.h:133-164:
+    // STACK_SIZE frames, 0 = one flat frame). Every promise carries a level in
+    // [0, LEVELS): make_work's promises start at level 0, and an async call
+    // places its promise at the caller's level plus a depth delta
+    // (async_call_cast(depth_delta, promise)), clamped to the stack. Each
+    // level has its own open links in the stash and its own link queues in
+    // device memory, and groups run and pull the deepest level first, as the
+    // reference's pull_promises_any_level does (program/async.h:1164-1181),
+    // which keeps the population of a recursive program bounded; a
+    // specification with SHALLOW_FIRST runs the shallowest level first.
+    static constexpr uint32_t LEVELS = STACK_SIZE == 0 ? 1u : (uint32_t)STACK_SIZE;
+    static_assert(LEVELS <= 16, "at most 16 depth levels (STACK_SIZE)");
+    static constexpr bool SHALLOW_FIRST = hz::spec::SHALLOW_FIRSTOr<PROGRAM_SPEC, 0>::value != 0;
 
     static const unsigned char FN_ID_COUNT = PromiseUnionType::Info::COUNT;
     static constexpr uint32_t OPS = PromiseUnionType::Info::COUNT;
+    static_assert(OPS <= 255, "at most 255 operations");
     static constexpr uint32_t WORDS = hz::PromiseWords<OpSet>::value;
 
     static_assert(std::is_trivially_copyable<PromiseUnionType>::value,
@@ -171,7 +297,7 @@ class AsyncProgram {
     // plus headroom.
     static constexpr uint32_t LINK_BYTES = WORDS * 64u * 4u;
     static constexpr uint32_t LDS_LINK_CAP = (48u * 1024u) / LINK_BYTES;
-    static constexpr uint32_t MIN_LINKS = OPS + 3u;
+    static constexpr uint32_t MIN_LINKS = OPS * LEVELS + 3u;
     static constexpr uint32_t WANT_LINKS =
         (STASH_SIZE < 32 ? (uint32_t)STASH_SIZE : 32u) < LDS_LINK_CAP ? (STASH_SIZE < 32 ? (uint32_t)STASH_SIZE : 32u)
                                                                       : LDS_LINK_CAP;
@@ -210,6 +336,10 @@ class AsyncProgram {
     struct ThreadContext {
         unsigned int thread_id;
         unsigned int rand_state;
+        // Promises this lane wrote into links. Counted per lane, in a
+        // register: appends run in divergent code, where a shared counter
+        // would need a cross-lane ordering on every call.
+        unsigned long long appended;
     };
 
     // Per-wavefront scheduler state, in LDS.
@@ -218,8 +348,10 @@ class AsyncProgram {
         uint32_t link_op[LINKS];
         uint32_t link_count[LINKS];
         uint32_t link_seq[OLDEST_FIRST ? LINKS : 1];  // fill order of ready links (OLDEST_FIRST)
+        uint32_t link_level[LEVELS > 1 ? LINKS : 1];  // depth level of each link's promises
         uint32_t seq;
-        uint32_t open[OPS > 0 ? OPS : 1];
+        uint32_t level;                                // level of the promises being evaluated
+        uint32_t open[OPS > 0 ? OPS * LEVELS : 1];     // open link of (operation, level)
         uint32_t free_mask;
         uint32_t ready_mask;
         uint32_t spare[SPARES];
@@ -265,15 +397,15 @@ class AsyncProgram {
     struct DeviceContext {
         typedef ProgramType ParentProgramType;
         hz::AsyncStatus* status;
-        hz::LinkQueue* queues;      // [queue_count]
-        unsigned int* queue_slots;  // [queue_count][queue_capacity]
+        hz::LinkQueue* queues;      // [LEVELS][queue_count]
+        unsigned int* queue_slots;  // [LEVELS][queue_count][queue_capacity]
         hz::LinkQueue* free_queue;
         unsigned int* free_slots;   // [queue_capacity]
         unsigned int* arena;        // [links][WORDS][64]
-        unsigned int* link_meta;    // [links]: op | count << 16
+        unsigned int* link_meta;    // [links]: op | level << 8 | count << 16
         unsigned int* link_aux;     // [links]: barrier bookkeeping (next link, fill)
         unsigned int links;
-        unsigned int queue_count;
+        unsigned int queue_count;   // link queues per depth level
         unsigned int queue_mask;    // queue_capacity - 1
         unsigned int coalesce;      // minimum fill before an open link runs while work can still be made
     };
@@ -315,8 +447,8 @@ class AsyncProgram {
               queue_capacity(pow2_at_least(arena_links < 1 ? 1 : arena_links)),
               coalesce(coalesce_min > 64 ? 64 : coalesce_min),
               status((size_t)1),
-              queues((size_t)queue_count),
-              queue_slots((size_t)queue_count * pow2_at_least(arena_links < 1 ? 1 : arena_links)),
+              queues((size_t)queue_count * LEVELS),
+              queue_slots((size_t)queue_count * LEVELS * pow2_at_least(arena_links < 1 ? 1 : arena_links)),
               free_queue((size_t)1),
               free_slots((size_t)pow2_at_least(arena_links < 1 ? 1 : arena_links)),
               arena((size_t)(arena_links < 1 ? 1 : arena_links) * WORDS * 64u),
@@ -356,6 +488,41 @@ class AsyncProgram {
 
         // True once every promise has been evaluated and make_work is exhausted.
         bool complete() { return status_complete(read_status()); }
+
+        // Host-side consistency check of the queues and the arena between
+        // launches (the role of the reference's runtime_overview,
+        // program/async.h:2766-2883); see hz::audit_snapshot. Call only while
+        // no launch of this instance runs. `allow_held` says whether the
+        // specification awaits on barriers at all.
+        hz::AsyncCensus audit(bool allow_held = false) {
+            hz::AsyncSnapshot snap;
+            snap.status = read_status();
+            snap.capacity = queue_capacity;
+            snap.arena_links = arena_size;
+            snap.ops = OPS;
+            snap.levels = LEVELS;
+            const size_t rings = (size_t)queue_count * LEVELS;
+            snap.queues.resize(rings + 1u);
+            util::host::auto_throw(hipMemcpy(snap.queues.data(), queues.data(), sizeof(hz::LinkQueue) * rings,
+                                             hipMemcpyDeviceToHost));
+            util::host::auto_throw(hipMemcpy(&snap.queues[rings], free_queue.data(), sizeof(hz::LinkQueue),
+                                             hipMemcpyDeviceToHost));
+            snap.slots.resize((rings + 1u) * queue_capacity);
+            util::host::auto_throw(hipMemcpy(snap.slots.data(), queue_slots.data(),
+                                             sizeof(unsigned int) * rings * queue_capacity, hipMemcpyDeviceToHost));
+            util::host::auto_throw(hipMemcpy(snap.slots.data() + rings * queue_capacity, free_slots.data(),
+                                             sizeof(unsigned int) * queue_capacity, hipMemcpyDeviceToHost));
+            const unsigned int claimed = snap.status.claim < arena_size ? snap.status.claim : arena_size;
+            snap.meta.resize(claimed);
+            snap.aux.resize(2 * (size_t)claimed);
+            if (claimed) {
+                util::host::auto_throw(hipMemcpy(snap.meta.data(), link_meta.data(), sizeof(unsigned int) * claimed,
+                                                 hipMemcpyDeviceToHost));
+                util::host::auto_throw(hipMemcpy(snap.aux.data(), link_aux.data(), 2 * sizeof(unsigned int) * claimed,
+                                                 hipMemcpyDeviceToHost));
+            }
+            return hz::audit_snapshot(snap, allow_held);
+        }
 
         void clear_flags() {
             unsigned int zero = 0;
@@ -476,9 +643,23 @@ class AsyncProgram {
         return id;
     }
 
-    __device__ hz::LinkQueue* work_queue(unsigned int q) { return _dev_ctx.queues + q; }
-    __device__ HZ_INLINE unsigned int* work_slots(unsigned int q) {
-        return _dev_ctx.queue_slots + (size_t)q * (_dev_ctx.queue_mask + 1u);
+    // Ring r = level * queue_count + queue (level 0 rings are 0..queue_count-1).
+    __device__ hz::LinkQueue* work_queue(unsigned int r) { return _dev_ctx.queues + r; }
+    __device__ HZ_INLINE unsigned int* work_slots(unsigned int r) {
+        return _dev_ctx.queue_slots + (size_t)r * (_dev_ctx.queue_mask + 1u);
+    }
+    __device__ HZ_INLINE unsigned int ring_of(unsigned int level, unsigned int q) {
+        return LEVELS > 1 ? level * _dev_ctx.queue_count + q : q;
+    }
+    // The i-th level in scheduling order (deepest first unless SHALLOW_FIRST).
+    __device__ static HZ_INLINE uint32_t level_in_order(uint32_t i) { return SHALLOW_FIRST ? i : LEVELS - 1u - i; }
+    __device__ HZ_INLINE uint32_t link_level_of(uint32_t L) {
+        if constexpr (LEVELS > 1) return hz::wave::first(_grp_ctx.link_level[L]);
+        return 0u;
+    }
+    // Index of stash link L's (operation, level) in the open-link table.
+    __device__ HZ_INLINE uint32_t open_index(uint32_t L) {
+        return hz::wave::first(_grp_ctx.link_op[L]) * LEVELS + link_level_of(L);
     }
 
     // ------------------------------------------------------------ arena links
@@ -554,13 +735,15 @@ class AsyncProgram {
         }
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
-        if (r == 0) _dev_ctx.link_meta[id] = op | (cnt << 16);
+        uint32_t lv = link_level_of(L);
+        if (r == 0) _dev_ctx.link_meta[id] = op | (lv << 8) | (cnt << 16);
         // One release for the link's words and metadata; the id is then
         // stored without a second one.
         hz::wave::fence_release();
         if (r == 0) {
             add_units(1);
-            queue_push(work_queue(g.home), work_slots(g.home), id, false);
+            const uint32_t ring = ring_of(lv, g.home);
+            queue_push(work_queue(ring), work_slots(ring), id, false);
             g.free_mask |= (1u << L);
             g.ready_mask &= ~(1u << L);
             g.spills += 1;
@@ -610,10 +793,14 @@ class AsyncProgram {
             unsigned int id = hz::LINK_NONE;
             if (lane == 0) {
                 uint32_t qn = _dev_ctx.queue_count;
-                for (uint32_t i = 0; i < qn && id == hz::LINK_NONE; ++i) {
-                    uint32_t q = g.home + i;
-                    if (q >= qn) q -= qn;
-                    id = queue_pop(work_queue(q), work_slots(q));
+                for (uint32_t li = 0; li < LEVELS && id == hz::LINK_NONE; ++li) {
+                    const uint32_t lv = level_in_order(li);
+                    for (uint32_t i = 0; i < qn && id == hz::LINK_NONE; ++i) {
+                        uint32_t q = g.home + i;
+                        if (q >= qn) q -= qn;
+                        const uint32_t ring = ring_of(lv, q);
+                        id = queue_pop(work_queue(ring), work_slots(ring));
+                    }
                 }
             }
             id = hz::wave::first(id);
@@ -625,7 +812,8 @@ class AsyncProgram {
             for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][lane] = src[w * 64u + lane];
             uint32_t meta = hz::wave::first(_dev_ctx.link_meta[id]);
             if (lane == 0) {
-                g.link_op[L] = meta & 0xFFFFu;
+                g.link_op[L] = meta & 0xFFu;
+                if constexpr (LEVELS > 1) g.link_level[L] = (meta >> 8) & 0xFFu;
                 g.link_count[L] = meta >> 16;
                 g.free_mask = fm & ~(1u << L);
                 g.ready_mask |= (1u << L);
@@ -644,20 +832,24 @@ class AsyncProgram {
         return pulled;
     }
 
-    // Appends one promise per active lane to the open link of operation D.
-    __device__ HZ_INLINE void stash_append(uint32_t D, const uint32_t* words) {
+    // Appends one promise per active lane to the open link of operation D at
+    // depth level lv (the same for every active lane).
+    __device__ HZ_INLINE void stash_append(uint32_t D, uint32_t lv, const uint32_t* words) {
         GroupContext& g = _grp_ctx;
         uint64_t act = hz::wave::active();
         uint32_t n = hz::wave::popc(act);
         uint32_t r = hz::wave::rank(act);
-        uint32_t L = hz::wave::first(g.open[D]);
+        const uint32_t slot = D * LEVELS + lv;
+        uint32_t L = hz::wave::first(g.open[slot]);
         uint32_t c = 0;
+        _thd_ctx.appended += 1u;
         if (L == hz::LINK_NONE) {
             L = take_link();
             if (L == hz::LINK_NONE) return;
             g.link_op[L] = D;
+            if constexpr (LEVELS > 1) g.link_level[L] = lv;
             g.link_count[L] = 0;
-            g.open[D] = L;
+            g.open[slot] = L;
             hz::wave::sync();
         } else {
             c = hz::wave::first(g.link_count[L]);
@@ -686,7 +878,7 @@ class AsyncProgram {
             g.link_seq[L] = q;
             g.seq = q + 1u;
         }
-        g.open[D] = hz::LINK_NONE;
+        g.open[slot] = hz::LINK_NONE;
         hz::wave::sync();
         uint32_t rest = n - room;
         if (rest == 0) return;
@@ -697,21 +889,47 @@ class AsyncProgram {
             for (uint32_t w = 0; w < WORDS; ++w) g.words[L2][w][r - room] = words[w];
         }
         g.link_op[L2] = D;
+        if constexpr (LEVELS > 1) g.link_level[L2] = lv;
         g.link_count[L2] = rest;
-        g.open[D] = L2;
+        g.open[slot] = L2;
         hz::wave::sync();
     }
 
+    // Depth level of a promise queued with `depth_delta` by the promise being
+    // evaluated (or by make_work, at level 0), clamped to the stack.
+    __device__ HZ_INLINE uint32_t child_level(int depth_delta) {
+        if constexpr (LEVELS == 1) {
+            return 0u;
+        } else {
+            int lv = (int)hz::wave::first(_grp_ctx.level) + depth_delta;
+            lv = lv < 0 ? 0 : lv;
+            return lv >= (int)LEVELS ? LEVELS - 1u : (uint32_t)lv;
+        }
+    }
+
+   public:
+    // Queues a promise at the current depth level plus depth_delta (reference
+    // async_call_cast, program/async.h:1627-1660); async<Op>(...) is the
+    // depth_delta = 0 form. A delta is meaningful only with STACK_SIZE levels.
     template <typename TYPE>
-    __device__ HZ_INLINE void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
+    __device__ HZ_INLINE void async_call_cast(int depth_delta, Promise<TYPE> param_value) {
         static_assert(Lookup<TYPE>::type::CONTAINED, "operation is not part of this program's OpSet");
         constexpr OpDisc D = Lookup<TYPE>::type::DISC;
         PromiseUnionType pu;
         pu.template cast<TYPE>() = param_value;
         uint32_t words[WORDS];
         __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
-        stash_append(D, words);
+        stash_append(D, child_level(depth_delta), words);
     }
+
+    // Depth level of the promise being evaluated (0 in make_work and without
+    // STACK_SIZE levels).
+    __device__ HZ_INLINE unsigned int depth() {
+        if constexpr (LEVELS == 1) return 0u;
+        return hz::wave::first(_grp_ctx.level);
+    }
+
+   protected:
 
     template <typename TYPE>
     __device__ HZ_INLINE void immediate_call_cast(Promise<TYPE> param_value) {
@@ -719,20 +937,46 @@ class AsyncProgram {
     }
 
     // Stash link with the most promises among the open ones, or LINK_NONE.
+    // With depth levels, the open links of the first level (in scheduling
+    // order) that has any are considered.
     __device__ HZ_INLINE uint32_t fullest_open(uint32_t& count) {
         GroupContext& g = _grp_ctx;
         uint32_t best = hz::LINK_NONE, best_count = 0;
-        for (uint32_t d = 0; d < OPS; ++d) {
-            uint32_t L = hz::wave::first(g.open[d]);
-            if (L == hz::LINK_NONE) continue;
-            uint32_t c = hz::wave::first(g.link_count[L]);
-            if (c > best_count) {
-                best = L;
-                best_count = c;
+        for (uint32_t li = 0; li < LEVELS && best == hz::LINK_NONE; ++li) {
+            const uint32_t lv = level_in_order(li);
+            for (uint32_t d = 0; d < OPS; ++d) {
+                uint32_t L = hz::wave::first(g.open[d * LEVELS + lv]);
+                if (L == hz::LINK_NONE) continue;
+                uint32_t c = hz::wave::first(g.link_count[L]);
+                if (c > best_count) {
+                    best = L;
+                    best_count = c;
+                }
             }
         }
         count = best_count;
         return best;
+    }
+
+    // The ready link of `mask` (non-empty) whose level comes first in
+    // scheduling order (depth levels), lowest stash position among equals.
+    __device__ HZ_INLINE uint32_t first_by_level(uint32_t mask) {
+        uint32_t best_link = lowest(mask), best = 0xFFFFFFFFu;
+        for (uint32_t m = mask; m != 0u; m &= m - 1u) {
+            uint32_t i = lowest(m);
+            uint32_t lv = link_level_of(i);
+            uint32_t rank = SHALLOW_FIRST ? lv : LEVELS - 1u - lv;
+            if (rank < best) {
+                best = rank;
+                best_link = i;
+            }
+        }
+        return best_link;
+    }
+
+    __device__ HZ_INLINE uint32_t pick_ready(uint32_t mask) {
+        if constexpr (LEVELS > 1) return first_by_level(mask);
+        return OLDEST_FIRST ? oldest(mask) : lowest(mask);
     }
 
     __device__ bool stash_empty() { return hz::wave::first(_grp_ctx.free_mask) == ALL_LINKS; }
@@ -759,6 +1003,7 @@ class AsyncProgram {
             g.made += 1;
         }
         hz::wave::sync();
+        if constexpr (LEVELS > 1) g.level = 0u;
         bool mine;
         [[clang::always_inline]] mine = PROGRAM_SPEC::make_work(*this);
         bool more = hz::wave::ballot(mine) != 0;
@@ -778,6 +1023,32 @@ class AsyncProgram {
         const uint32_t lane = hz::wave::lane();
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
+#ifdef HZ_RUN_ONE_REGION
+        {
+            HZ_PHASE(PHASE_READ);
+            uint32_t fm = hz::wave::first(g.free_mask);
+            uint32_t rmask = hz::wave::first(g.ready_mask);
+            g.free_mask = fm | (1u << L);
+            g.ready_mask = rmask & ~(1u << L);
+            g.executed += cnt;
+            g.runs += 1;
+            if constexpr (LEVELS > 1) g.level = link_level_of(L);
+        }
+        {
+            HZ_PHASE(PHASE_EVAL);
+            if (lane < cnt) {
+                uint32_t words[WORDS];
+#pragma unroll
+                for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+                PromiseUnionType pu;
+                __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
+                pu.template loose_eval<ProgramType>(*this, op);
+            }
+            hz::wave::sync();
+        }
+        drop_holding_if_empty();
+        return;
+#endif
         uint32_t words[WORDS];
         {
             HZ_PHASE(PHASE_READ);
@@ -794,6 +1065,7 @@ class AsyncProgram {
             g.ready_mask = rmask & ~(1u << L);
             g.executed += cnt;
             g.runs += 1;
+            if constexpr (LEVELS > 1) g.level = link_level_of(L);
             hz::wave::sync();
         }
         {
@@ -808,9 +1080,15 @@ class AsyncProgram {
         drop_holding_if_empty();
     }
 
+    // True if the group's home queue holds links (at some depth level).
     __device__ HZ_INLINE bool queues_hint() {
-        hz::LinkQueue* q = work_queue(hz::wave::first(_grp_ctx.home));
-        return hz::wave::first((int32_t)hz::wave::atomic_load(&q->count)) > 0;
+        const uint32_t home = hz::wave::first(_grp_ctx.home);
+        int32_t any = 0;
+        for (uint32_t lv = 0; lv < LEVELS && any <= 0; ++lv) {
+            hz::LinkQueue* q = work_queue(ring_of(lv, home));
+            any = hz::wave::first((int32_t)hz::wave::atomic_load(&q->count));
+        }
+        return any > 0;
     }
 
     enum CycleResult { RUN = 0, DONE = 1 };
@@ -854,7 +1132,7 @@ class AsyncProgram {
                 L = lowest(rm);
             } else {
                 uint32_t best = 0;
-                for (uint32_t d = 0; d < OPS; ++d) {
+                for (uint32_t d = 0; d < OPS * LEVELS; ++d) {
                     uint32_t o = hz::wave::first(g.open[d]);
                     if (o == hz::LINK_NONE || o == keep) continue;
                     uint32_t c = hz::wave::first(g.link_count[o]);
@@ -864,8 +1142,7 @@ class AsyncProgram {
                     }
                 }
                 if (L != hz::LINK_NONE) {
-                    uint32_t op = hz::wave::first(g.link_op[L]);
-                    g.open[op] = hz::LINK_NONE;
+                    g.open[open_index(L)] = hz::LINK_NONE;
                     hz::wave::sync();
                 }
             }
@@ -921,13 +1198,12 @@ class AsyncProgram {
         {
             HZ_PHASE(PHASE_PICK);
             if (rm != 0) {
-                L = OLDEST_FIRST ? oldest(rm) : lowest(rm);
+                L = pick_ready(rm);
             } else {
                 uint32_t c = 0;
                 L = fullest_open(c);
                 if (L != hz::LINK_NONE) {
-                    uint32_t op = hz::wave::first(g.link_op[L]);
-                    g.open[op] = hz::LINK_NONE;
+                    g.open[open_index(L)] = hz::LINK_NONE;
                     hz::wave::sync();
                 }
             }
@@ -938,6 +1214,15 @@ class AsyncProgram {
             set_hungry(false);
             share_links(L);
             run_link(L);
+#if defined(HZ_RUN_BURST) && HZ_RUN_BURST > 1
+            for (uint32_t b = 1; b < HZ_RUN_BURST; ++b) {
+                uint32_t rm2 = hz::wave::first(g.ready_mask);
+                if (rm2 == 0) break;
+                uint32_t fm2 = hz::wave::first(g.free_mask);
+                if ((uint32_t)__builtin_popcount(fm2) < MARGIN) break;
+                run_link(pick_ready(rm2));
+            }
+#endif
             return RUN;
         }
 
@@ -987,7 +1272,8 @@ class AsyncProgram {
             g.free_mask = ALL_LINKS;
             g.ready_mask = 0;
             g.seq = 0;
-            for (uint32_t d = 0; d < OPS; ++d) g.open[d] = hz::LINK_NONE;
+            g.level = 0;
+            for (uint32_t d = 0; d < OPS * LEVELS; ++d) g.open[d] = hz::LINK_NONE;
             g.spare_count = 0;
             g.holding = 0;
             g.can_make_work = 1;
@@ -1010,6 +1296,7 @@ class AsyncProgram {
     __device__ HZ_INLINE void init_thread() {
         _thd_ctx.thread_id = blockIdx.x * blockDim.x + threadIdx.x;
         _thd_ctx.rand_state = _thd_ctx.thread_id;
+        _thd_ctx.appended = 0;
     }
 
     // Leaves nothing in LDS: every stash link goes back to the arena
@@ -1018,7 +1305,7 @@ class AsyncProgram {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         if (lane == 0) {
-            for (uint32_t d = 0; d < OPS; ++d) g.open[d] = hz::LINK_NONE;
+            for (uint32_t d = 0; d < OPS * LEVELS; ++d) g.open[d] = hz::LINK_NONE;
         }
         hz::wave::sync();
         while (true) {
@@ -1031,6 +1318,10 @@ class AsyncProgram {
                 break;
             }
         }
+        // Sum the lanes' append counts (every lane is active here).
+        unsigned long long appended = _thd_ctx.appended;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) appended += __shfl_xor(appended, d, 64);
         if (lane == 0) {
             if (g.hungry) {
                 g.hungry = 0;
@@ -1052,6 +1343,7 @@ class AsyncProgram {
             hz::AsyncStatus* st = _dev_ctx.status;
             hz::wave::atomic_add(&st->promises, g.executed);
             hz::wave::atomic_add(&st->links_run, g.runs);
+            if (appended) hz::wave::atomic_add(&st->appended, appended);
             if (g.spills) hz::wave::atomic_add(&st->spills, g.spills);
             if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
             if (g.made) hz::wave::atomic_add(&st->make_work_calls, g.made);
@@ -1137,6 +1429,7 @@ class AsyncProgram {
         uint32_t words[WORDS];
         __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
         unsigned long long* part = &b.partial_table[D];
+        bool held = false;  // this lane's promise went into a barrier link
         for (uint32_t attempt = 0; hz::wave::ballot(pending) != 0ull; ++attempt) {
             if (attempt >= SLOT_SPIN_LIMIT) {
                 if (pending) stall(hz::STALL_BARRIER_WAIT, hz::wave::atomic_load(part));
@@ -1168,6 +1461,7 @@ class AsyncProgram {
                 } else if (L != hz::BARRIER_NO_LINK && s < 64u) {
                     barrier_write(b, L, s, D, words);
                     pending = false;
+                    held = true;
                 } else if ((L == hz::BARRIER_NO_LINK && s == 0u) || (L != hz::BARRIER_NO_LINK && s == 64u)) {
                     // This lane installs a fresh link with its promise in slot 0.
                     pending = false;
@@ -1200,6 +1494,7 @@ class AsyncProgram {
                         }
                         if (installed) {
                             barrier_finish(b, fresh, D);
+                            held = true;
                         } else if (released) {  // released meanwhile: run it now
                             queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh, false);
                             run_now = true;
@@ -1212,6 +1507,7 @@ class AsyncProgram {
             }
             if (hz::wave::ballot(pending) != 0ull) hz::wave::sleep_short();
         }
+        if (held) _thd_ctx.appended += 1u;
         if (run_now) {
             // Pairs with the release of the semaphore update that released
             // the barrier, so the promise sees what was written before it.
@@ -1341,10 +1637,11 @@ class AsyncProgram {
         size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
         size_t stride = (size_t)gridDim.x * blockDim.x;
         size_t cap = (size_t)ctx.queue_mask + 1u;
-        size_t nslots = cap * ctx.queue_count;
+        const size_t rings = (size_t)ctx.queue_count * LEVELS;
+        size_t nslots = cap * rings;
         for (size_t i = tid; i < nslots; i += stride) ctx.queue_slots[i] = hz::LINK_NONE;
         for (size_t i = tid; i < cap; i += stride) ctx.free_slots[i] = hz::LINK_NONE;
-        for (size_t i = tid; i < ctx.queue_count; i += stride) {
+        for (size_t i = tid; i < rings; i += stride) {
             ctx.queues[i].tail = 0;
             ctx.queues[i].head = 0;
             ctx.queues[i].count = 0;
@@ -1364,6 +1661,7 @@ class AsyncProgram {
             st->claim = 0;
             st->promises = st->spills = st->fills = st->make_work_calls = st->launches = st->idle_polls = 0;
             st->links_run = 0;
+            st->appended = 0;
             for (int i = 0; i < 8; ++i) st->phase_cycles[i] = 0;
         }
     }

@@ -218,9 +218,12 @@ class EventProgram {
 
     __device__ void set_flags(unsigned int bits) { hz::wave::atomic_or(&_dev_ctx.status->flags, bits); }
 
+   public:
     // Appends one promise per active lane to the output queue of TYPE: one
     // atomic per wavefront on the home segment, spilling to the following
-    // segments only when it is full.
+    // segments only when it is full. The depth delta is accepted and ignored:
+    // a generation runs every queued promise, whatever its depth (as in the
+    // reference, program/event.h:326-358).
     template <typename TYPE>
     __device__ void async_call_cast(int /*depth_delta*/, Promise<TYPE> param_value) {
         constexpr OpDisc D = Lookup<TYPE>::type::DISC;
@@ -254,6 +257,11 @@ class EventProgram {
         if (hz::wave::ballot(pending) != 0 && hz::wave::leader()) set_flags(OVERFLOW_FLAG);
     }
 
+    // Depth level of the promise being evaluated: a level-synchronous
+    // program keeps no levels, so always 0.
+    __device__ unsigned int depth() { return 0u; }
+
+   protected:
     template <typename TYPE>
     __device__ void immediate_call_cast(Promise<TYPE> param_value) {
         param_value(*this);

@@ -38,7 +38,7 @@ void EventProgram<PROGRAM_SPEC>::Instance::reset(hipStream_t stream) {
 
 template <typename PROGRAM_SPEC>
 void AsyncProgram<PROGRAM_SPEC>::Instance::reset(hipStream_t stream) {
-    size_t slots = (size_t)queue_capacity * (queue_count + 1u);
+    size_t slots = (size_t)queue_capacity * ((size_t)queue_count * AsyncProgram<PROGRAM_SPEC>::LEVELS + 1u);
     size_t blocks = (slots + 255) / 256;
     if (blocks > 1024) blocks = 1024;
     if (blocks == 0) blocks = 1;

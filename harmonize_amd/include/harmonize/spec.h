@@ -57,6 +57,9 @@ HZ_CONST_OR(SHARE_WORK)
 // supersede earlier ones (a BFS level lowered again) wastes less. Not a
 // reference constant.
 HZ_CONST_OR(OLDEST_FIRST)
+// Nonzero, with STACK_SIZE depth levels: run and pull the shallowest level
+// first instead of the deepest. Not a reference constant.
+HZ_CONST_OR(SHALLOW_FIRST)
 #undef HZ_CONST_OR
 
 // The operation set a program type executes: the specification's OpSet, plus

@@ -296,6 +296,14 @@ typedef struct hz_run_config {
 } hz_run_config;
 
 #define HZ_RUN_GRAPH 1u
+/* Audit the AsyncProgram's queues and arena on the host after every launch
+ * (a debugging aid, the role of the reference's runtime_overview,
+ * program/async.h:2766-2883): every group has left, ring counters agree with
+ * their slots, no link id is queued twice or both queued and free, the units
+ * count equals the queued links, and every promise written into a link is
+ * evaluated, queued or held by a barrier. A violation fails the run with
+ * HZ_ERR_RUNTIME naming it. */
+#define HZ_RUN_AUDIT 2u
 
 typedef struct hz_run_stats {
     uint64_t promises;        /* promises evaluated */
@@ -304,7 +312,8 @@ typedef struct hz_run_stats {
     uint64_t fills;           /* async: links moved HBM -> LDS */
     uint64_t make_work_calls; /* wavefront calls of make_work */
     uint32_t groups;          /* groups actually launched */
-    uint32_t reserved;
+    uint32_t audits;          /* async, HZ_RUN_AUDIT: launches after which
+                                 the queues and arena were audited */
     double elapsed_ms;        /* hipEvent time of the exec loop */
     uint64_t phase_cycles[8]; /* async, runtime built with HZ_PHASE_TIMING:
                                  shader cycles summed over groups per phase
@@ -313,12 +322,31 @@ typedef struct hz_run_stats {
     uint64_t links_run;       /* async: links evaluated; promises / links_run
                                  is the mean number of busy lanes per
                                  evaluation (64 = perfect coalescing) */
+    uint64_t arena_links;     /* async: arena links ever taken from the bump
+                                 allocator (the high-water mark of the arena) */
 } hz_run_stats;
 
 /* Runs the Collatz program over `count` starts (d_starts may be NULL: start
  * = index) and writes d_steps[count]. kind = HZ_EVENT, HZ_ASYNC or HZ_PLAIN. */
 int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count, uint32_t* d_steps,
                    const hz_run_config* config, hz_run_stats* stats);
+
+/* Depth levels (the reference's non-flat work stack, mem/work.h:133-164,
+ * and depth-aware async calls, program/async.h:1164-1181, 1627-1660): a tree
+ * of promises where each Node at height d < height queues `fanout` children
+ * one depth level deeper and each leaf counts itself for its root, so
+ * d_leaves[r] = fanout^height for every root r. `order` picks the program:
+ *   HZ_TREE_FLAT        one level (STACK_SIZE 0);
+ *   HZ_TREE_DEEPEST     8 levels, deepest first (the reference's order);
+ *   HZ_TREE_SHALLOWEST  8 levels, shallowest first.
+ * *d_violations counts nodes whose prog.depth() differed from their height
+ * clamped to the levels. stats->arena_links shows how many arena links the
+ * order needed. kind = HZ_ASYNC, or HZ_EVENT with HZ_TREE_FLAT. */
+#define HZ_TREE_FLAT 0
+#define HZ_TREE_DEEPEST 1
+#define HZ_TREE_SHALLOWEST 2
+int hz_tree_run(int kind, int order, uint32_t roots, uint32_t fanout, uint32_t height, uint32_t* d_leaves,
+                uint32_t* d_violations, const hz_run_config* config, hz_run_stats* stats);
 
 /* Allocation-free variant of hz_collatz_run(HZ_ASYNC, ...). It has no
  * counterpart in the reference: it is for callers that cannot allocate device
@@ -562,6 +590,15 @@ int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, 
  * that many bytes (out[i] = offset, or all-ones if it does not fit);
  * ops[i] = -k - 1 frees the k-th allocation (out[i] = largest free block). */
 int hz_pool_test(uint64_t size, const int64_t* ops, uint32_t count, uint64_t* out);
+
+/* Host-only check of the HZ_RUN_AUDIT audit itself: it must accept a
+ * consistent snapshot of queues and arena and reject each of a set of planted
+ * inconsistencies (a link queued twice, ring counters off, a stray slot, the
+ * units count off, a lost promise, a leaked link, a bad operation, an empty
+ * queued link, a group still checked in, an invalid id). Returns the number
+ * of plants that went undetected (0 = all caught), or a negative status;
+ * `report` receives a summary. */
+int hz_audit_selftest(char* report, size_t report_bytes);
 
 /* Device facts used to size launches. */
 typedef struct hz_device_facts {

@@ -11,8 +11,13 @@
 //  - an operation is dispatched by its discriminant, the position of its
 //    type in OpSet (async/promise.h:65-91, :647-658);
 //  - every promise is evaluated exactly once; sync<Op> evaluates in place
-//    (program/event.h:361-367).
-// Promises are evaluated first-in first-out, one at a time.
+//    (program/event.h:361-367);
+//  - a promise's depth level is its creator's level plus the call's depth
+//    delta, clamped to the specification's STACK_SIZE levels (make_work's
+//    promises are at level 0), and prog.depth() reports it
+//    (program/async.h:1395-1411, mem/work.h:133-164).
+// Promises are evaluated first-in first-out, one at a time (the order does
+// not change a deterministic specification's results).
 //
 // OmpProgram<Spec>: the same contract on all host threads. Each thread has
 // a private queue and its own group/thread state; surplus promises move to
@@ -56,7 +61,15 @@ class HostProgram {
     using GroupState = typename hz::spec::GroupStateOr<SPEC, VoidState>::type;
     using ThreadState = typename hz::spec::ThreadStateOr<SPEC, VoidState>::type;
     typedef PromiseUnion<OpSet> PromiseUnionType;
-    using Entry = std::pair<OpDisc, PromiseUnionType>;
+    // A queued promise: discriminant, payload and depth level.
+    struct Entry {
+        OpDisc first;
+        PromiseUnionType second;
+        uint32_t level;
+        Entry(OpDisc d, const PromiseUnionType& p, uint32_t lv = 0) : first(d), second(p), level(lv) {}
+    };
+    static constexpr uint32_t LEVELS =
+        hz::spec::STACK_SIZEOr<SPEC, 0>::value == 0 ? 1u : (uint32_t)hz::spec::STACK_SIZEOr<SPEC, 0>::value;
 
     template <typename TYPE>
     struct Lookup {
@@ -68,6 +81,7 @@ class HostProgram {
     struct Runner {
         std::deque<Entry> queue;
         RunStats stats;
+        uint32_t level = 0;  // level of the promise being evaluated
     };
 
     DeviceState& device;
@@ -85,12 +99,21 @@ class HostProgram {
 
     template <typename TYPE>
     void async_call(Promise<TYPE> promise) {
+        async_call_cast<TYPE>(0, promise);
+    }
+
+    template <typename TYPE>
+    void async_call_cast(int depth_delta, Promise<TYPE> promise) {
         static_assert(Lookup<TYPE>::type::CONTAINED, "operation is not part of this program's OpSet");
         PromiseUnionType pu;
         pu.template cast<TYPE>() = promise;
-        _run.queue.emplace_back(Lookup<TYPE>::type::DISC, pu);
+        int lv = (int)_run.level + depth_delta;
+        lv = lv < 0 ? 0 : (lv >= (int)LEVELS ? (int)LEVELS - 1 : lv);
+        _run.queue.emplace_back(Lookup<TYPE>::type::DISC, pu, (uint32_t)lv);
         if (_run.queue.size() > _run.stats.peak_queue) _run.stats.peak_queue = _run.queue.size();
     }
+
+    unsigned int depth() const { return _run.level; }
 
     template <typename TYPE, typename... ARGS>
     void sync(ARGS... args) {
@@ -170,6 +193,7 @@ class HostProgram {
         if (_run.queue.empty()) return false;
         Entry e = _run.queue.front();
         _run.queue.pop_front();
+        _run.level = e.level;
         e.second.template loose_eval<ProgramType>(*this, e.first);
         _run.stats.promises++;
         return true;
@@ -188,6 +212,7 @@ class HostProgram {
             if (prog.step()) continue;
             if (can_make) {
                 r.stats.make_work_calls++;
+                r.level = 0;
                 can_make = SPEC::make_work(prog);
                 continue;
             }
@@ -240,6 +265,7 @@ struct OmpProgram {
                 }
                 if (can_make) {
                     r.stats.make_work_calls++;
+                    r.level = 0;
                     can_make = SPEC::make_work(prog);
                     continue;
                 }

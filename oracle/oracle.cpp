@@ -14,6 +14,7 @@
 #include "barrier_demo.h"
 #include "futures.h"
 #include "bfs.h"
+#include "tree.h"
 #include "philox.h"
 #include "rmat.h"
 
@@ -211,6 +212,35 @@ int oracle_barrier_program(uint32_t dep, uint32_t non_dep, uint32_t barrier_coun
     auto t0 = std::chrono::steady_clock::now();
     oracle::RunStats s = used == 1 ? oracle::HostProgram<BarrierDemoSpec>::run(st)
                                    : oracle::OmpProgram<BarrierDemoSpec>::run(st, used);
+    auto t1 = std::chrono::steady_clock::now();
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Runs the depth-level tree specification on the host executors (same
+// outputs as hz_tree_run): order 0 = flat, 1 = 8 levels deepest first, 2 = 8
+// levels shallowest first. The host executors evaluate in their own order;
+// the levels they report follow the depth deltas of the calls.
+int oracle_tree_program(int order, uint32_t roots, uint32_t fanout, uint32_t height, uint32_t* leaves,
+                        uint32_t* violations, int threads, oracle_run_stats* stats) {
+    util::iter::AtomicIter<unsigned int> iter(0u, roots);
+    *violations = 0;
+    for (uint32_t i = 0; i < roots; ++i) leaves[i] = 0;
+    TreeState st{roots, fanout, height, order == 0 ? 1u : TREE_LEVELS, &iter, leaves, violations};
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s;
+    auto go = [&](auto* spec_tag) {
+        using S = std::remove_pointer_t<decltype(spec_tag)>;
+        s = used == 1 ? oracle::HostProgram<S>::run(st) : oracle::OmpProgram<S>::run(st, used);
+    };
+    if (order == 0) {
+        go((TreeFlatSpec*)nullptr);
+    } else if (order == 1) {
+        go((TreeDeepSpec*)nullptr);
+    } else {
+        go((TreeShallowSpec*)nullptr);
+    }
     auto t1 = std::chrono::steady_clock::now();
     fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
     return 0;

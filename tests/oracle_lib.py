@@ -124,6 +124,23 @@ def barrier_program(dep, non_dep, barriers, threads=1):
     return int(counter[0]), hits[:dep], int(violations[0]), st
 
 
+_SIGS["oracle_tree_program"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP,
+                                                _VP, ctypes.c_int, ctypes.POINTER(OracleStats)])
+
+TREE_FLAT, TREE_DEEPEST, TREE_SHALLOWEST = 0, 1, 2
+TREE_LEVELS = 8
+
+
+def tree_program(order, roots, fanout, height, threads=1):
+    """Runs the depth-level tree specification on the host executors; returns (leaves, violations, stats)."""
+    leaves = np.zeros(max(1, roots), dtype=np.uint32)
+    violations = np.zeros(1, dtype=np.uint32)
+    st = OracleStats()
+    lib().oracle_tree_program(order, roots, fanout, height, leaves.ctypes.data, violations.ctypes.data, threads,
+                              ctypes.byref(st))
+    return leaves[:roots], int(violations[0]), st
+
+
 _SIGS["oracle_futures_program"] = (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int,
                                                    ctypes.POINTER(OracleStats)])
 

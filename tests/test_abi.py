@@ -152,3 +152,11 @@ def test_noalloc_refuses_sizes_beyond_its_static_array_before_any_launch(field, 
     rc = hz.lib().hz_collatz_run_noalloc(None, 0, None, ctypes.byref(cfg), ctypes.byref(st))
     assert rc == -1
     assert "static maximum" in hz.last_error()
+
+
+def test_audit_catches_planted_inconsistencies():
+    # The HZ_RUN_AUDIT check accepts a consistent snapshot of queues and arena
+    # and names each planted fault (host code only, no GPU).
+    buf = ctypes.create_string_buffer(4096)
+    missed = hz.lib().hz_audit_selftest(buf, len(buf))
+    assert missed == 0, buf.value.decode()

@@ -46,7 +46,8 @@ def test_gpu_held_links_survive_relaunch():
     import harmonize_amd as hz
 
     dep, non_dep, barriers = 100000, 1000, 16
-    counter, hits, violations, st = hz.barrier_run(dep, non_dep, barriers, cycles=64, groups=256,
-                                                   max_launches=100000)
+    counter, hits, violations, st = hz.barrier_run(dep, non_dep, barriers, cycles=4, groups=64,
+                                                   max_launches=100000, flags=hz.RUN_AUDIT)
     assert counter == 2 * dep + non_dep
+    assert st["launches"] > 1 and st["audits"] == st["launches"]
     assert (hits == 1).all() and violations == 0

@@ -67,8 +67,8 @@ def test_levels_match_textbook(hz, small_batch, kind, cfg):
 
 def test_relaunch_resumes(hz, small_batch):
     b, want = small_batch
-    st = b.run("async", groups=32, cycles=8, max_launches=1 << 20)
-    assert st["launches"] > 1
+    st = b.run("async", groups=32, cycles=8, max_launches=1 << 20, flags=hz.RUN_AUDIT)
+    assert st["launches"] > 1 and st["audits"] == st["launches"]
     assert np.array_equal(b.levels(), want)
 
 

@@ -86,10 +86,13 @@ def test_async_relaunch_and_spill_path():
     host = O.collatz_make_starts(11, 40, n)
     ref, promises = O.collatz_steps(host)
     dev = hz.DeviceArray.from_numpy(host)
-    got, st = run("async", n, starts=dev, cycles=500, groups=512, arena=1 << 16, max_launches=100000)
+    got, st = run("async", n, starts=dev, cycles=500, groups=512, arena=1 << 16, max_launches=100000,
+                  flags=hz.RUN_AUDIT)
     assert np.array_equal(got, ref)
     assert st["promises"] == promises
     assert st["launches"] > 1 and st["spills"] > 0 and st["fills"] == st["spills"]
+    # the host audit of queues, arena and promise accounts after every launch
+    assert st["audits"] == st["launches"]
 
 
 @pytest.mark.parametrize("queues", [1, 3, 16])

@@ -61,8 +61,9 @@ def test_async_relaunches_and_small_grid():
     # A small cycle budget ends launches early: held Echo / ReturnOp links and
     # stash links must survive the relaunches.
     starts = np.arange(1, (1 << 14) + 1, dtype=np.uint64)
-    total, echo, forwarded, st = _run("async", starts, 3000, groups=64, cycles=256, max_launches=100000)
-    assert st["launches"] > 1
+    total, echo, forwarded, st = _run("async", starts, 3000, groups=64, cycles=256, max_launches=100000,
+                                      flags=2)  # harmonize_amd.RUN_AUDIT
+    assert st["launches"] > 1 and st["audits"] == st["launches"]
     _check_against_oracle(starts, 3000, total, echo, forwarded, st)
 
 

@@ -97,8 +97,9 @@ def test_async_relaunch_and_spill_path():
     n = 200000
     ref_t, ref_e, _ = O.photon_program(O.photon_params(photons=n), threads=0)
     tally, events, st = hz.photon_run("async", hz.photon_params(photons=n), cycles=40, groups=512,
-                                      max_launches=100000)
+                                      max_launches=100000, flags=hz.RUN_AUDIT)
     assert st["launches"] > 1 and st["spills"] > 0
+    assert st["audits"] == st["launches"]
     assert np.array_equal(events, ref_e)
     assert_tally_close(tally, ref_t)
 
