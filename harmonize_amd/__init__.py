@@ -104,6 +104,11 @@ _SIGNATURES = {
     ),
     "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
     "hz_audit_selftest": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "hz_chase_run": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _VP, _VP, _VP, _VP, ctypes.POINTER(RunConfig),
+         ctypes.POINTER(RunStats)],
+    ),
     "hz_tree_run": (
         ctypes.c_int,
         [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP,
@@ -335,6 +340,20 @@ def barrier_run(dep_count: int, non_dep_count: int, barrier_count: int, **cfg):
 
 
 TREE_FLAT, TREE_DEEPEST, TREE_SHALLOWEST = 0, 1, 2
+
+
+def chase_run(kind, lazy: bool, heads: DeviceArray, nodes: DeviceArray, **cfg):
+    """Runs the pointer-chasing program over lists starting at `heads` (u32)
+    through `nodes` (u32 pairs [next, value]); returns (sums, lengths, stats)."""
+    k = KINDS[kind] if isinstance(kind, str) else int(kind)
+    lists = heads.count
+    sums = DeviceArray(max(1, lists), np.uint64)
+    lengths = DeviceArray(max(1, lists), np.uint32)
+    st = RunStats()
+    config = make_config(**cfg)
+    check(lib().hz_chase_run(k, int(bool(lazy)), lists, heads.ptr, nodes.ptr, sums.ptr, lengths.ptr,
+                             ctypes.byref(config), ctypes.byref(st)), "hz_chase_run")
+    return sums.numpy()[:lists], lengths.numpy()[:lists], st.as_dict()
 
 
 def tree_run(kind, order: int, roots: int, fanout: int, height: int, **cfg):

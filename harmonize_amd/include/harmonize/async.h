@@ -307,6 +307,42 @@ class AsyncProgram {
     // Free links kept before a link is evaluated: one per operation for the
     // links its children may open, plus one.
     static constexpr uint32_t MARGIN = OPS + 1u;
+    // Ready links a group may run per scheduling cycle. Running up to four
+    // took Collatz 2^26 from 70.7 to 67.0 ms and the photon walk from 5.78 to
+    // 5.66 ms; a program that shares work checks its hungry count once per
+    // cycle, so it runs one.
+    static constexpr uint32_t BURST = SHARE_WORK ? 1u : 4u;
+
+    // Asynchronous argument loads (reference ASYNC_LOADS, async/promise.h:
+    // 284-303, program/async.h:2082-2144): an operation whose parameters
+    // include a LazyLoad<T> has the T of each promise loaded before it runs.
+    // When a group starts a link, it issues direct-to-LDS loads
+    // (global_load_lds) of the lazy values of the ready link it will most
+    // likely run next, lane i of that link into column i of a per-group
+    // staging block, and then evaluates the current link while they are in
+    // flight; the next link reads its values from the block. A link that was
+    // not prefetched reads its values when it runs.
+    template <unsigned I>
+    struct LazyFacts {
+        using Op = typename hz::OpOf<I, OpSet>::type;
+        using First = hz::FirstLazy<typename Promise<Op>::Args>;
+        static constexpr bool HAS = First::HAS;
+        static constexpr uint32_t VALUE_WORDS = HAS ? (uint32_t)(sizeof(typename First::Value) / 4u) : 0u;
+        static_assert(!HAS || sizeof(typename First::Value) % 4u == 0, "a LazyLoad value must be whole 32-bit words");
+    };
+    template <unsigned I = 0>
+    static constexpr uint32_t lazy_words() {
+        if constexpr (I >= OPS) {
+            return 0u;
+        } else {
+            constexpr uint32_t here = LazyFacts<I>::VALUE_WORDS;
+            constexpr uint32_t rest = lazy_words<I + 1>();
+            return here > rest ? here : rest;
+        }
+    }
+    static constexpr uint32_t LAZY_WORDS = lazy_words();
+    static constexpr bool HAS_LAZY = LAZY_WORDS > 0;
+    static_assert(LAZY_WORDS <= 16, "a LazyLoad value may span at most 16 words");
     static constexpr uint32_t SPARES = 4;
 
     static const unsigned int BAD_FUNC_ID_FLAG = 0x00000001;
@@ -371,6 +407,7 @@ class AsyncProgram {
         uint32_t share_check;  // cycles until the hungry count is consulted again
         uint32_t hungry;       // this group is counted in status->hungry
         uint32_t no_makers;    // this group has seen (and acquired) makers == 0
+        uint32_t pf_link;      // stash link whose lazy values are being loaded (HAS_LAZY)
 #ifdef HZ_PHASE_TIMING
         unsigned long long int phase[8];
 #endif
@@ -563,6 +600,11 @@ class AsyncProgram {
     GroupState& group;
     ThreadState& thread;
 
+    // The group's staging block for prefetched lazy values, [word][lane]
+    // (a __shared__ object of its own, so that waiting for the loads into it
+    // does not hold up the stash's LDS traffic).
+    uint32_t (*_lazy_stage)[64] = nullptr;
+
     __device__ AsyncProgram(DeviceContext& d_c, GroupContext& g_c, ThreadContext& t_c, DeviceState& d,
                             GroupState& g, ThreadState& t)
         : _dev_ctx(d_c), _grp_ctx(g_c), _thd_ctx(t_c), device(d), group(g), thread(t) {}
@@ -736,6 +778,9 @@ class AsyncProgram {
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
         uint32_t lv = link_level_of(L);
+        if constexpr (HAS_LAZY) {
+            if (hz::wave::first(g.pf_link) == L) g.pf_link = hz::LINK_NONE;
+        }
         if (r == 0) _dev_ctx.link_meta[id] = op | (lv << 8) | (cnt << 16);
         // One release for the link's words and metadata; the id is then
         // stored without a second one.
@@ -1018,43 +1063,90 @@ class AsyncProgram {
         drop_holding_if_empty();
     }
 
+    // ------------------------------------------------- lazy argument loads
+    // Word offset of operation I's first LazyLoad member (its source pointer
+    // and its value) inside a promise.
+    template <unsigned I>
+    __device__ static HZ_INLINE uint32_t lazy_source_word() {
+        PromiseUnionType pu;
+        auto& z = LazyFacts<I>::First::of(pu.template cast<typename LazyFacts<I>::Op>().args);
+        return (uint32_t)(((const char*)&z.source - (const char*)&pu) / 4);
+    }
+
+    // Issues the direct-to-LDS loads of stash link N's lazy values (operation
+    // I), lane i's into column i of the staging block.
+    template <unsigned I = 0>
+    __device__ HZ_INLINE void prefetch_lazy(uint32_t N, uint32_t op, uint32_t cnt) {
+        if constexpr (I < OPS) {
+            if constexpr (LazyFacts<I>::HAS) {
+                if (op == I) {
+                    const uint32_t lane = hz::wave::lane();
+                    const uint32_t sw = lazy_source_word<I>();
+                    if (lane < cnt) {
+                        const uint64_t lo = _grp_ctx.words[N][sw][lane];
+                        const uint64_t hi = _grp_ctx.words[N][sw + 1u][lane];
+                        const uint32_t* src = (const uint32_t*)(uintptr_t)((hi << 32) | lo);
+#pragma unroll
+                        for (uint32_t k = 0; k < LazyFacts<I>::VALUE_WORDS; ++k) {
+                            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + k),
+                                                             (__attribute__((address_space(3))) void*)&_lazy_stage[k][0],
+                                                             4, 0, 0);
+                        }
+                    }
+                    return;
+                }
+            }
+            prefetch_lazy<I + 1>(N, op, cnt);
+        }
+    }
+
+    // Hands this lane's prefetched lazy value to its promise (operation I).
+    template <unsigned I = 0>
+    __device__ HZ_INLINE void take_lazy(PromiseUnionType& pu, uint32_t op, const uint32_t* staged) {
+        if constexpr (I < OPS) {
+            if constexpr (LazyFacts<I>::HAS) {
+                if (op == I) {
+                    auto& z = LazyFacts<I>::First::of(pu.template cast<typename LazyFacts<I>::Op>().args);
+                    typename LazyFacts<I>::First::Value v;
+                    __builtin_memcpy(&v, staged, sizeof(v));
+                    z.fill(v);
+                    return;
+                }
+            }
+            take_lazy<I + 1>(pu, op, staged);
+        }
+    }
+
+    template <unsigned I = 0>
+    __device__ static HZ_INLINE bool op_has_lazy(uint32_t op) {
+        if constexpr (I < OPS) {
+            return (op == I && LazyFacts<I>::HAS) || op_has_lazy<I + 1>(op);
+        }
+        return false;
+    }
+
     __device__ HZ_INLINE void run_link(uint32_t L) {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
         uint32_t op = hz::wave::first(g.link_op[L]);
         uint32_t cnt = hz::wave::first(g.link_count[L]);
-#ifdef HZ_RUN_ONE_REGION
-        {
-            HZ_PHASE(PHASE_READ);
-            uint32_t fm = hz::wave::first(g.free_mask);
-            uint32_t rmask = hz::wave::first(g.ready_mask);
-            g.free_mask = fm | (1u << L);
-            g.ready_mask = rmask & ~(1u << L);
-            g.executed += cnt;
-            g.runs += 1;
-            if constexpr (LEVELS > 1) g.level = link_level_of(L);
-        }
-        {
-            HZ_PHASE(PHASE_EVAL);
-            if (lane < cnt) {
-                uint32_t words[WORDS];
-#pragma unroll
-                for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
-                PromiseUnionType pu;
-                __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
-                pu.template loose_eval<ProgramType>(*this, op);
-            }
-            hz::wave::sync();
-        }
-        drop_holding_if_empty();
-        return;
-#endif
         uint32_t words[WORDS];
+        uint32_t staged[HAS_LAZY ? LAZY_WORDS : 1];
+        bool prefetched = false;
         {
             HZ_PHASE(PHASE_READ);
             if (lane < cnt) {
 #pragma unroll
                 for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+            }
+            if constexpr (HAS_LAZY) {
+                prefetched = hz::wave::first(g.pf_link) == L;
+                if (prefetched && lane < cnt) {
+#pragma unroll
+                    for (uint32_t k = 0; k < LAZY_WORDS; ++k) staged[k] = _lazy_stage[k][lane];
+                }
+                // The staging block is read before the next link's loads refill it.
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
             }
             hz::wave::sync();
             // All lanes store the same values (uniform read-modify-writes of
@@ -1066,6 +1158,20 @@ class AsyncProgram {
             g.executed += cnt;
             g.runs += 1;
             if constexpr (LEVELS > 1) g.level = link_level_of(L);
+            if constexpr (HAS_LAZY) {
+                // Start the loads of the link most likely to run next.
+                uint32_t next = hz::LINK_NONE;
+                const uint32_t rest = rmask & ~(1u << L);
+                if (rest != 0u) {
+                    const uint32_t N = pick_ready(rest);
+                    const uint32_t nop = hz::wave::first(g.link_op[N]);
+                    if (op_has_lazy(nop)) {
+                        prefetch_lazy(N, nop, hz::wave::first(g.link_count[N]));
+                        next = N;
+                    }
+                }
+                g.pf_link = next;
+            }
             hz::wave::sync();
         }
         {
@@ -1073,6 +1179,9 @@ class AsyncProgram {
             if (lane < cnt) {
                 PromiseUnionType pu;
                 __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
+                if constexpr (HAS_LAZY) {
+                    if (prefetched) take_lazy(pu, op, staged);
+                }
                 pu.template loose_eval<ProgramType>(*this, op);
             }
             hz::wave::sync();
@@ -1214,15 +1323,15 @@ class AsyncProgram {
             set_hungry(false);
             share_links(L);
             run_link(L);
-#if defined(HZ_RUN_BURST) && HZ_RUN_BURST > 1
-            for (uint32_t b = 1; b < HZ_RUN_BURST; ++b) {
+            // Further ready links run in the same cycle while the margin of
+            // free links holds, skipping the cycle's other checks.
+            for (uint32_t b = 1; b < BURST; ++b) {
                 uint32_t rm2 = hz::wave::first(g.ready_mask);
                 if (rm2 == 0) break;
                 uint32_t fm2 = hz::wave::first(g.free_mask);
                 if ((uint32_t)__builtin_popcount(fm2) < MARGIN) break;
                 run_link(pick_ready(rm2));
             }
-#endif
             return RUN;
         }
 
@@ -1273,6 +1382,7 @@ class AsyncProgram {
             g.ready_mask = 0;
             g.seq = 0;
             g.level = 0;
+            g.pf_link = hz::LINK_NONE;
             for (uint32_t d = 0; d < OPS * LEVELS; ++d) g.open[d] = hz::LINK_NONE;
             g.spare_count = 0;
             g.holding = 0;
@@ -1304,6 +1414,9 @@ class AsyncProgram {
     __device__ HZ_INLINE void cleanup_runtime() {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
+        // No load into the staging block may still be in flight when the
+        // workgroup's LDS is released.
+        if constexpr (HAS_LAZY) __builtin_amdgcn_s_waitcnt(0);
         if (lane == 0) {
             for (uint32_t d = 0; d < OPS * LEVELS; ++d) g.open[d] = hz::LINK_NONE;
         }
@@ -1629,6 +1742,10 @@ class AsyncProgram {
         ThreadContext tctx;
         ThreadState thread_state;
         ProgramType prog(ctx, gctx[g], tctx, state, group_state[g], thread_state);
+        if constexpr (HAS_LAZY) {
+            __shared__ uint32_t lazy_stage[GROUPS_PER_BLOCK][LAZY_WORDS][64];
+            prog._lazy_stage = lazy_stage[g];
+        }
         prog.exec(cycles);
     }
 

@@ -151,14 +151,35 @@ struct ReturnOp {
     __device__ HZ_INLINE static void eval(PROGRAM program, TaggedSemaphore* sem, void* dst, void* src, size_t size);
 };
 
-// Marks an argument that the reference loads lazily (async/promise.h:204-240).
-// Here the value is read eagerly when the promise is built.
+// An argument filled in from device memory (reference LazyLoad,
+// async/promise.h:204-240, and ASYNC_LOADS, :284-303). It is built at the
+// call site from a reference to the value in device memory.
+//  - Passed where the operation takes a plain TYPE (the reference's form
+//    without ASYNC_LOADS), it converts at once: the value is read when the
+//    promise is built.
+//  - Declared as the operation's parameter (`LazyLoad<TYPE>`), the promise
+//    carries the address. AsyncProgram loads the values of a whole link with
+//    direct-to-LDS loads issued while the link before it is evaluated, and
+//    hands them to the operation in `value`; elsewhere, or if the link was
+//    not prefetched, get() reads the source when called.
+// TYPE must be trivially copyable, 4-byte aligned and a whole number of
+// 32-bit words. Bit 0 of `source` (free, by that alignment) marks a filled
+// `value`.
 template <typename TYPE>
 struct LazyLoad {
     using Type = TYPE;
-    Type& reference;
-    HZ_HD LazyLoad(Type& ref) : reference(ref) {}
-    HZ_HD operator TYPE() { return reference; }
+    static_assert(alignof(TYPE) >= 4 && sizeof(TYPE) % 4 == 0, "LazyLoad values are whole aligned 32-bit words");
+    uintptr_t source;
+    TYPE value;
+    LazyLoad() = default;
+    HZ_HD LazyLoad(const TYPE& ref) : source((uintptr_t)&ref), value() {}
+    HZ_HD TYPE get() const { return (source & 1u) ? value : *(const TYPE*)source; }
+    HZ_HD operator TYPE() const { return get(); }
+    // Hands over the loaded value.
+    HZ_HD void fill(const TYPE& v) {
+        value = v;
+        source |= 1u;
+    }
 };
 
 template <typename... TYPES>
@@ -195,6 +216,35 @@ struct ArgTuple<HEAD, NEXT, TAIL...> {
     ArgTuple() = default;
     HZ_HD ArgTuple(HEAD h, NEXT n, TAIL... t) : head(h), tail(n, t...) {}
 };
+
+namespace hz {
+// The first LazyLoad<T> parameter of an argument tuple, if any: HAS, the
+// loaded type Value, and of(t), the member itself.
+template <typename TUPLE>
+struct FirstLazy {
+    static constexpr bool HAS = false;
+    using Value = unsigned int;
+};
+template <typename T>
+struct FirstLazy<ArgTuple<LazyLoad<T>>> {
+    static constexpr bool HAS = true;
+    using Value = T;
+    static HZ_HD LazyLoad<T>& of(ArgTuple<LazyLoad<T>>& t) { return t.head; }
+};
+template <typename T, typename N, typename... R>
+struct FirstLazy<ArgTuple<LazyLoad<T>, N, R...>> {
+    static constexpr bool HAS = true;
+    using Value = T;
+    static HZ_HD LazyLoad<T>& of(ArgTuple<LazyLoad<T>, N, R...>& t) { return t.head; }
+};
+template <typename H, typename N, typename... R>
+struct FirstLazy<ArgTuple<H, N, R...>> {
+    using Rest = FirstLazy<ArgTuple<N, R...>>;
+    static constexpr bool HAS = Rest::HAS;
+    using Value = typename Rest::Value;
+    static HZ_HD auto& of(ArgTuple<H, N, R...>& t) { return Rest::of(t.tail); }
+};
+}  // namespace hz
 
 // Deduces the return and argument types of an operation from `using Type = R(*)(Args...)`.
 template <typename TYPE>

@@ -348,6 +348,23 @@ int hz_collatz_run(int kind, const uint64_t* d_starts, uint32_t count, uint32_t*
 int hz_tree_run(int kind, int order, uint32_t roots, uint32_t fanout, uint32_t height, uint32_t* d_leaves,
                 uint32_t* d_violations, const hz_run_config* config, hz_run_stats* stats);
 
+/* Asynchronous argument loads (the reference's LazyLoad and ASYNC_LOADS,
+ * async/promise.h:204-240, 284-303; program/async.h:2082-2144), on linked
+ * lists: list l starts at node d_heads[l] (HZ_CHASE_END: empty) and follows
+ * d_nodes[i].next until HZ_CHASE_END; d_sums[l] = sum of its nodes' values,
+ * d_lengths[l] = its node count. Each step consumes its node as a LazyLoad
+ * argument: lazy = 1 declares the parameter LazyLoad<node>, so AsyncProgram
+ * loads a link's nodes with direct-to-LDS loads while the link before it
+ * runs; lazy = 0 declares a plain node, read when the promise is built.
+ * kind = HZ_ASYNC or HZ_EVENT (which reads lazy nodes when it evaluates). */
+#define HZ_CHASE_END 0xFFFFFFFFu
+typedef struct hz_chase_node {
+    uint32_t next;
+    uint32_t value;
+} hz_chase_node;
+int hz_chase_run(int kind, int lazy, uint32_t lists, const uint32_t* d_heads, const hz_chase_node* d_nodes,
+                 uint64_t* d_sums, uint32_t* d_lengths, const hz_run_config* config, hz_run_stats* stats);
+
 /* Allocation-free variant of hz_collatz_run(HZ_ASYNC, ...). It has no
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code that sets up HIP graph capture, tight serving loops, harnesses

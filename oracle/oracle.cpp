@@ -15,6 +15,7 @@
 #include "futures.h"
 #include "bfs.h"
 #include "tree.h"
+#include "chase.h"
 #include "philox.h"
 #include "rmat.h"
 
@@ -240,6 +241,25 @@ int oracle_tree_program(int order, uint32_t roots, uint32_t fanout, uint32_t hei
         go((TreeDeepSpec*)nullptr);
     } else {
         go((TreeShallowSpec*)nullptr);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);
+    return 0;
+}
+
+// Runs the pointer-chasing specification (lazy = LazyLoad parameter or
+// plain node) on the host executors (same outputs as hz_chase_run).
+int oracle_chase_program(int lazy, uint32_t lists, const uint32_t* heads, const uint32_t* nodes, uint64_t* sums,
+                         uint32_t* lengths, int threads, oracle_run_stats* stats) {
+    util::iter::AtomicIter<unsigned int> iter(0u, lists);
+    ChaseState st{lists, 0u, heads, (const ChaseNode*)nodes, (unsigned long long*)sums, lengths, &iter};
+    int used = threads == 1 ? 1 : (threads <= 0 ? omp_get_max_threads() : threads);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::RunStats s;
+    if (lazy) {
+        s = used == 1 ? oracle::HostProgram<ChaseLazySpec>::run(st) : oracle::OmpProgram<ChaseLazySpec>::run(st, used);
+    } else {
+        s = used == 1 ? oracle::HostProgram<ChaseEagerSpec>::run(st) : oracle::OmpProgram<ChaseEagerSpec>::run(st, used);
     }
     auto t1 = std::chrono::steady_clock::now();
     fill_stats(stats, s, std::chrono::duration<double, std::milli>(t1 - t0).count(), used);

@@ -141,6 +141,50 @@ def tree_program(order, roots, fanout, height, threads=1):
     return leaves[:roots], int(violations[0]), st
 
 
+_SIGS["oracle_chase_program"] = (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, _VP, _VP, _VP, _VP, ctypes.c_int,
+                                                 ctypes.POINTER(OracleStats)])
+CHASE_END = 0xFFFFFFFF
+
+
+def chase_lists(nodes, lists, seed):
+    """Random linked lists: a permutation of `nodes` node ids cut into `lists`
+    ragged runs (some empty); returns (heads[lists] u32, node table
+    [nodes][next, value] u32, sums u64, lengths u32). The sums and lengths are
+    the closed form: each run's values summed in order."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(nodes).astype(np.uint32)
+    cuts = np.sort(rng.integers(0, nodes + 1, size=max(0, lists - 1)))
+    bounds = np.concatenate([[0], cuts, [nodes]]).astype(np.int64)
+    values = rng.integers(0, 1 << 32, size=nodes, dtype=np.uint64).astype(np.uint32)
+    table = np.zeros((nodes, 2), dtype=np.uint32)
+    table[:, 1] = values
+    heads = np.full(lists, CHASE_END, dtype=np.uint32)
+    sums = np.zeros(lists, dtype=np.uint64)
+    lengths = (bounds[1:] - bounds[:-1]).astype(np.uint32)
+    for l in range(lists):
+        a, b = bounds[l], bounds[l + 1]
+        if b > a:
+            run = perm[a:b]
+            heads[l] = run[0]
+            table[run[:-1], 0] = run[1:]
+            table[run[-1], 0] = CHASE_END
+            sums[l] = values[run].astype(np.uint64).sum()
+    return heads, table, sums, lengths
+
+
+def chase_program(lazy, heads, table, threads=1):
+    """Runs the pointer-chasing specification on the host executors; returns (sums, lengths, stats)."""
+    lists = len(heads)
+    heads = np.ascontiguousarray(heads, dtype=np.uint32)
+    table = np.ascontiguousarray(table, dtype=np.uint32)
+    sums = np.zeros(max(1, lists), dtype=np.uint64)
+    lengths = np.zeros(max(1, lists), dtype=np.uint32)
+    st = OracleStats()
+    lib().oracle_chase_program(int(lazy), lists, heads.ctypes.data, table.ctypes.data, sums.ctypes.data,
+                               lengths.ctypes.data, threads, ctypes.byref(st))
+    return sums[:lists], lengths[:lists], st
+
+
 _SIGS["oracle_futures_program"] = (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP, _VP, _VP, ctypes.c_int,
                                                    ctypes.POINTER(OracleStats)])
 
