@@ -60,7 +60,7 @@ def algorithmic_bytes(promises: int, starts: int) -> int:
 # reads for a 1 GiB device-to-device copy (`bench.py --leg copy`): FETCH_SIZE
 # reads 512 MiB for the copy's 1 GiB of reads (factor 2.0) and WRITE_SIZE
 # reads 1 GiB for its 1 GiB of writes (factor 1.0).
-TRAFFIC_KIB = {"fetch": 1037835.9, "write": 2219237.9}
+TRAFFIC_KIB = {"fetch": 1034111.5, "write": 2218712.7}
 TRAFFIC_FACTOR = {"fetch": 2.0, "write": 1.0}
 
 
@@ -68,31 +68,63 @@ def measured_traffic_bytes() -> int:
     return int(sum(TRAFFIC_KIB[k] * TRAFFIC_FACTOR[k] * 1024 for k in TRAFFIC_KIB))
 
 
-def cpu_baseline(threads_all: int) -> dict:
+# Instruction counts of one AsyncProgram exec launch on this workload, from a
+# `rocprofv3 --pmc SQ_INSTS_SALU SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_BRANCH
+# SQ_WAVES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT` pass over
+# `bench.py --leg collatz_async` (totals over all wavefronts of the dispatch).
+ISSUE_COUNTS = {"salu": 2.621e10, "valu": 1.528e10, "lds": 7.735e9, "branch": 9.863e9}
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
+LDS_BYTES_PER_CLOCK_PER_CU = 128
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo, as lscpu reports it)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(repeats: int = 5) -> dict:
+    """The oracle's host executors on a bounded sample of the headline workload:
+    one warm-up, then the median of `repeats` timed runs each (steady_clock
+    inside the executor)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the oracle, used only as the CPU baseline leg
 
+    def median_rate(starts, threads):
+        O.collatz_program(starts, threads=threads)
+        runs = [O.collatz_program(starts, threads=threads)[1] for _ in range(repeats)]
+        rates = sorted(r.promises / (r.elapsed_ms * 1e-3) for r in runs)
+        return rates[len(rates) // 2], runs[0]
+
     sample = 1 << 22
     starts = O.collatz_make_starts(SEED, BITS, COUNT)[:sample]
-    _, st_omp = O.collatz_program(starts, threads=0)
-    omp_rate = st_omp.promises / (st_omp.elapsed_ms * 1e-3)
-    seq_sample = 1 << 20
-    _, st_seq = O.collatz_program(starts[:seq_sample], threads=1)
-    seq_rate = st_seq.promises / (st_seq.elapsed_ms * 1e-3)
+    omp_rate, st_omp = median_rate(starts, 0)
+    seq_sample = 1 << 19
+    seq_rate, st_seq = median_rate(starts[:seq_sample], 1)
     return {
         "value": omp_rate,
         "unit": "promises/s",
         "cores": int(st_omp.threads),
         "kind": "port",
         "sample": f"first {sample} of the 2^26 starts, Collatz spec on the oracle's OpenMP host executor "
-                  f"({st_omp.promises} promises, {st_omp.elapsed_ms / 1e3:.1f} s)",
+                  f"({st_omp.promises} promises, {st_omp.elapsed_ms / 1e3:.2f} s per run); "
+                  f"median of {repeats} runs after one warm-up",
         "sequential": {
             "value": seq_rate,
             "cores": 1,
             "sample": f"first {seq_sample} starts on the sequential host executor "
-                      f"({st_seq.promises} promises, {st_seq.elapsed_ms / 1e3:.1f} s)",
+                      f"({st_seq.promises} promises, {st_seq.elapsed_ms / 1e3:.2f} s per run); "
+                      f"median of {repeats} runs after one warm-up",
         },
-        "cpu_model": platform.processor() or platform.machine(),
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
     }
 
 
@@ -441,8 +473,20 @@ def main() -> int:
     kernel_ms = float(np.mean([r["elapsed_ms"] for r in runs]))
     launches = int(np.mean([r["launches"] for r in runs]))
     per_launch_ms = kernel_ms / max(1, launches)
-    alg = algorithmic_bytes(promises, COUNT) / max(1, launches)
-    achieved = alg / (per_launch_ms * 1e-3) / 1e9
+    per_launch_s = per_launch_ms * 1e-3
+    cus = facts["compute_units"]
+    # The bound the headline kernel reaches: scalar instruction issue. A CU's
+    # four SIMDs share one scalar unit that issues at most one instruction per
+    # clock, so the chip's peak is CUs x clock.
+    salu_peak = cus * CLOCK_GHZ  # G SALU instructions/s
+    salu_achieved = ISSUE_COUNTS["salu"] / max(1, launches) / per_launch_s / 1e9
+    # Beside it: the LDS bytes the stash moves (each promise is written into a
+    # link and read back out, 2 x 16 B) against 128 B per CU per clock, and
+    # the measured HBM traffic against the HBM peak.
+    lds_achieved = 2 * PROMISE_BYTES * promises / max(1, launches) / per_launch_s / 1e9
+    lds_peak = cus * LDS_BYTES_PER_CLOCK_PER_CU * CLOCK_GHZ
+    hbm_achieved = measured_traffic_bytes() / per_launch_s / 1e9
+    alg_equiv = algorithmic_bytes(promises, COUNT) / max(1, launches) / per_launch_s / 1e9
 
     out = {
         "metric": METRIC,
@@ -471,16 +515,27 @@ def main() -> int:
             "compute_units": facts["compute_units"],
         },
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
+            "bound": "salu",
+            "achieved": salu_achieved,
+            "peak": salu_peak,
+            "unit": "G SALU instructions/s",
+            "frac": salu_achieved / salu_peak,
             "traffic": measured_traffic_bytes(),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
             "kernel": "_hz_exec_kernel<AsyncProgram<CollatzSpec>>",
             "kernel_ms": per_launch_ms,
-            "bytes_rule": "2 x 16 B per promise (enqueue + dequeue of the packed payload) + 12 B per start",
+            "peak_model": f"one scalar instruction per CU per clock: {cus} CUs x {CLOCK_GHZ} GHz",
+            "counts_per_launch": {k: v / max(1, launches) for k, v in ISSUE_COUNTS.items()},
+            "counters": "profiles/collatz_async_issue_counters.md",
+            "lds": {"achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s", "frac": lds_achieved / lds_peak,
+                    "rule": "2 x 16 B per promise (stash write + read); peak 128 B per CU per clock"},
+            "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": hbm_achieved / HBM_PEAK_GBPS, "rule": "measured traffic per launch / launch time"},
+            "algorithmic_hbm_equivalent": {
+                "achieved": alg_equiv, "unit": "GB/s",
+                "rule": "2 x 16 B per promise + 12 B per start: the bytes an HBM-queued runtime would move "
+                        "for the same promises (informational; the stash keeps them in LDS)",
+            },
         },
     }
 
@@ -521,7 +576,7 @@ def main() -> int:
     }
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline()
 
     print(json.dumps(out), flush=True)
     return 0
