@@ -792,6 +792,11 @@ _SIGNATURES.update(
         "hz_state_load": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),
         "hz_state_store": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),
         "hz_set_device": (ctypes.c_int, [ctypes.c_int]),
+        "hz_init_on": (ctypes.c_int, [_VP, ctypes.c_uint32, _VP]),
+        "hz_exec_on": (ctypes.c_int, [_VP, ctypes.c_uint32, ctypes.c_uint32, _VP]),
+        "hz_complete_on": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_int)]),
+        "hz_stream_create": (ctypes.c_int, [ctypes.POINTER(_VP)]),
+        "hz_stream_destroy": (ctypes.c_int, [_VP]),
         "hz_serve_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),
         "hz_serve_poll": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     }
@@ -800,6 +805,17 @@ _SIGNATURES.update(
 
 class CollatzState(ctypes.Structure):
     _fields_ = [("starts", _VP), ("steps", _VP), ("iterator", _VP)]
+
+
+def stream_create():
+    """A new non-blocking HIP stream (an opaque handle)."""
+    s = _VP()
+    check(lib().hz_stream_create(ctypes.byref(s)), "hz_stream_create")
+    return s.value
+
+
+def stream_destroy(stream) -> None:
+    check(lib().hz_stream_destroy(stream), "hz_stream_destroy")
 
 
 class Instance:
@@ -822,6 +838,20 @@ class Instance:
     def complete(self) -> bool:
         d = ctypes.c_int()
         check(lib().hz_complete(self.handle, ctypes.byref(d)), "hz_complete")
+        return d.value == 1
+
+    def init_on(self, groups: int, stream) -> None:
+        """Enqueues init on `stream` (from stream_create) without waiting."""
+        check(lib().hz_init_on(self.handle, groups, stream), "hz_init_on")
+
+    def exec_on(self, groups: int, cycles: int, stream) -> None:
+        """Enqueues one launch on `stream` without waiting."""
+        check(lib().hz_exec_on(self.handle, groups, cycles, stream), "hz_exec_on")
+
+    def complete_on(self, stream) -> bool:
+        """Waits for `stream`, then reports completion."""
+        d = ctypes.c_int()
+        check(lib().hz_complete_on(self.handle, stream, ctypes.byref(d)), "hz_complete_on")
         return d.value == 1
 
     def load_state(self):

@@ -242,6 +242,22 @@ void run_event(typename P::Instance& inst, uint32_t groups, uint32_t chunk, uint
             hz::capi::check(hipDeviceSynchronize(), "exec_program");                                    \
         });                                                                                             \
     }                                                                                                   \
+    extern "C" void init_program_on_##suffix(void* instance_ptr, size_t grid_size, void* stream) {       \
+        hz::capi::guarded([&] {                                                                         \
+            auto h = (suffix##_handle*)instance_ptr;                                                    \
+            h->refresh();                                                                               \
+            init<PROG>(h->inst, grid_size, (hipStream_t)stream);                                        \
+            hz::capi::check(hipGetLastError(), "init_program launch");                                  \
+        });                                                                                             \
+    }                                                                                                   \
+    extern "C" void exec_program_on_##suffix(void* instance_ptr, size_t grid_size, size_t cycle_count,  \
+                                             void* stream) {                                            \
+        hz::capi::guarded([&] {                                                                         \
+            auto h = (suffix##_handle*)instance_ptr;                                                    \
+            exec<PROG>(h->inst, grid_size, cycle_count, (hipStream_t)stream);                           \
+            hz::capi::check(hipGetLastError(), "exec_program launch");                                  \
+        });                                                                                             \
+    }                                                                                                   \
     extern "C" int complete_##suffix(void* instance_ptr) {                                              \
         int result = -1;                                                                                \
         hz::capi::guarded([&] { result = ((suffix##_handle*)instance_ptr)->inst.complete() ? 1 : 0; });  \

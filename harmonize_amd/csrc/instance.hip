@@ -22,13 +22,15 @@ struct ProgramEntry {
     void (*exec_program)(void*, size_t, size_t);
     int (*complete)(void*);
     void (*clear_flags)(void*);
+    void (*init_on)(void*, size_t, void*);
+    void (*exec_on)(void*, size_t, size_t, void*);
 };
 
 #define HZ_ENTRY(suffix, STATE)                                                                           \
     ProgramEntry {                                                                                        \
         #suffix, sizeof(STATE), alloc_state_##suffix, free_state_##suffix, alloc_program_##suffix,        \
             free_program_##suffix, init_program_##suffix, exec_program_##suffix, complete_##suffix,        \
-            clear_flags_##suffix                                                                          \
+            clear_flags_##suffix, init_program_on_##suffix, exec_program_on_##suffix                      \
     }
 
 // [spec][kind], kind = HZ_EVENT or HZ_ASYNC; a null name marks a pair that does not exist.
@@ -37,7 +39,8 @@ const ProgramEntry* entry_for(int spec, int kind) {
         {HZ_ENTRY(collatz_event, hz_collatz_state), HZ_ENTRY(collatz_async, hz_collatz_state)},
         {HZ_ENTRY(photon_event, hz_photon_state), HZ_ENTRY(photon_async, hz_photon_state)},
         {HZ_ENTRY(bfs_event, hz_bfs_state), HZ_ENTRY(bfs_async, hz_bfs_state)},
-        {ProgramEntry{nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr},
+        {ProgramEntry{nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr},
          HZ_ENTRY(barrier_async, hz_barrier_state)},
     };
     if (spec < 0 || spec > 3 || (kind != HZ_EVENT && kind != HZ_ASYNC)) return nullptr;
@@ -136,6 +139,47 @@ extern "C" int hz_complete(hz_instance* inst, int* done) {
         int r = need(inst)->entry->complete(inst->program);
         rethrow_last("complete");
         *done = r;
+    });
+}
+
+extern "C" int hz_init_on(hz_instance* inst, uint32_t groups, void* stream) {
+    return guarded([&] {
+        need(inst)->entry->init_on(inst->program, groups, stream);
+        rethrow_last("init_program_on");
+    });
+}
+
+extern "C" int hz_exec_on(hz_instance* inst, uint32_t groups, uint32_t cycles, void* stream) {
+    return guarded([&] {
+        if (groups == 0) throw std::invalid_argument("hz_exec_on: groups must be positive");
+        need(inst)->entry->exec_on(inst->program, groups, cycles, stream);
+        rethrow_last("exec_program_on");
+    });
+}
+
+extern "C" int hz_complete_on(hz_instance* inst, void* stream, int* done) {
+    return guarded([&] {
+        if (!done) throw std::invalid_argument("hz_complete_on: null output");
+        need(inst);
+        check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        int r = inst->entry->complete(inst->program);
+        rethrow_last("complete");
+        *done = r;
+    });
+}
+
+extern "C" int hz_stream_create(void** stream) {
+    return guarded([&] {
+        if (!stream) throw std::invalid_argument("hz_stream_create: null output");
+        hipStream_t s = nullptr;
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        *stream = s;
+    });
+}
+
+extern "C" int hz_stream_destroy(void* stream) {
+    return guarded([&] {
+        if (stream) check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
     });
 }
 

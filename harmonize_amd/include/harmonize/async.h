@@ -209,8 +209,8 @@ enum AsyncStall : unsigned int {
     STALL_NONE = 0,
     STALL_QUEUE_PUSH = 1,     // a ring slot stayed occupied (detail: queue << 32 | slot)
     STALL_QUEUE_POP = 2,      // a reserved ring slot stayed empty (detail: queue << 32 | slot)
-    STALL_BARRIER_WAIT = 3,   // no link became available in a barrier (detail: partial word)
-    STALL_BARRIER_INSTALL = 4,  // a fresh barrier link could not be published (detail: partial word)
+    STALL_BARRIER_WAIT = 3,   // a barrier block's link did not appear (detail: its ring entry)
+    STALL_BARRIER_INSTALL = 4,  // a released barrier's last block link did not appear (detail: ring entry)
     STALL_BARRIER_LIST = 5,   // a full barrier link could not be listed (detail: list head)
 };
 
@@ -1186,7 +1186,9 @@ class AsyncProgram {
             }
             hz::wave::sync();
         }
+#ifndef HZ_DROP_AFTER_BURST
         drop_holding_if_empty();
+#endif
     }
 
     // True if the group's home queue holds links (at some depth level).
@@ -1332,6 +1334,9 @@ class AsyncProgram {
                 if ((uint32_t)__builtin_popcount(fm2) < MARGIN) break;
                 run_link(pick_ready(rm2));
             }
+#ifdef HZ_DROP_AFTER_BURST
+            drop_holding_if_empty();
+#endif
             return RUN;
         }
 
@@ -1534,89 +1539,60 @@ class AsyncProgram {
     template <typename BARRIER, typename OP_TYPE>
     __device__ HZ_INLINE void barrier_await(BARRIER& b, Promise<OP_TYPE> promise) {
         constexpr OpDisc D = Lookup<OP_TYPE>::type::DISC;
+        constexpr unsigned int RING = BARRIER::RING;
         bool run_now = hz::wave::atomic_load(&b.semaphore.sem) == 0u;
         bool pending = !run_now;
-        if (pending) hz::wave::atomic_add(&b.count, 1u);
+        unsigned int ticket = 0u;
+        if (pending) {
+            hz::wave::atomic_add(&b.count, 1u);
+            // One atomic per await; acquire pairs with the release that set
+            // RELEASED_BIT, after which the promise runs at once.
+            ticket = hz::wave::atomic_add_acqrel(&b.tickets[D], 1u);
+            if (ticket & BARRIER::RELEASED_BIT) {
+                run_now = true;
+                pending = false;
+            }
+        }
         PromiseUnionType pu;
         pu.template cast<OP_TYPE>() = promise;
         uint32_t words[WORDS];
         __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
-        unsigned long long* part = &b.partial_table[D];
+        const unsigned int block = ticket / 64u;
+        const unsigned int slot = ticket % 64u;
+        unsigned long long* entry = &b.ring[D][block % RING];
         bool held = false;  // this lane's promise went into a barrier link
         for (uint32_t attempt = 0; hz::wave::ballot(pending) != 0ull; ++attempt) {
             if (attempt >= SLOT_SPIN_LIMIT) {
-                if (pending) stall(hz::STALL_BARRIER_WAIT, hz::wave::atomic_load(part));
+                if (pending) stall(hz::STALL_BARRIER_WAIT, hz::wave::atomic_load(entry));
                 break;
             }
-            // While a link is being installed ((NO_LINK, s > 0) or (L, s > 64))
-            // a waiter only reads the word: reserving then would change it
-            // and starve the installer's compare-and-swap. (L, 64) is still
-            // reserved: the lane that draws 64 installs the next link.
-            bool reserve = false;
             if (pending) {
-                unsigned long long seen = hz::wave::atomic_load(part);
-                unsigned int L = (unsigned int)(seen >> 32);
-                unsigned int s = (unsigned int)seen;
-                if (L == hz::BARRIER_RELEASED) {
-                    run_now = true;
-                    pending = false;
-                } else {
-                    reserve = L == hz::BARRIER_NO_LINK ? s == 0u : s <= 64u;
-                }
-            }
-            if (reserve) {
-                unsigned long long old = hz::wave::atomic_add_acqrel(part, 1ull);
-                unsigned int L = (unsigned int)(old >> 32);
-                unsigned int s = (unsigned int)old;
-                if (L == hz::BARRIER_RELEASED) {
-                    run_now = true;
-                    pending = false;
-                } else if (L != hz::BARRIER_NO_LINK && s < 64u) {
-                    barrier_write(b, L, s, D, words);
-                    pending = false;
-                    held = true;
-                } else if ((L == hz::BARRIER_NO_LINK && s == 0u) || (L != hz::BARRIER_NO_LINK && s == 64u)) {
-                    // This lane installs a fresh link with its promise in slot 0.
-                    pending = false;
-                    unsigned int fresh = arena_alloc_any();
-                    if (fresh == hz::LINK_NONE) {
-                        set_flags(ARENA_EXHAUSTED_FLAG);
-                    } else {
-                        hz::wave::atomic_store(&_dev_ctx.link_aux[2u * fresh + 1u], 64u << 16);
-                        write_arena_slot(fresh, 0u, words);
-                        // The count word must be set before a lane that reads
-                        // the installed link from the partial word counts into it.
-                        hz::wave::fence_release();
-                        unsigned long long want = ((unsigned long long)fresh << 32) | 1ull;
-                        unsigned long long cur = hz::wave::atomic_load(part);
-                        bool installed = false, released = false;
-                        // Only this lane installs until its link is published;
-                        // the word changes meanwhile only by reservations that
-                        // were already in flight, or by a release.
-                        for (uint32_t tries = 0; tries < SLOT_SPIN_LIMIT; ++tries) {
-                            if ((unsigned int)(cur >> 32) == hz::BARRIER_RELEASED) {
-                                released = true;
-                                break;
-                            }
-                            unsigned long long seen = atomicCAS(part, cur, want);
-                            if (seen == cur) {
-                                installed = true;
-                                break;
-                            }
-                            cur = seen;
-                        }
-                        if (installed) {
-                            barrier_finish(b, fresh, D);
-                            held = true;
-                        } else if (released) {  // released meanwhile: run it now
-                            queue_push(_dev_ctx.free_queue, _dev_ctx.free_slots, fresh, false);
-                            run_now = true;
+                const unsigned long long seen = hz::wave::atomic_load_acquire(entry);
+                if (slot == 0u) {
+                    // This lane installs the block's link once the block RING
+                    // before it has filled and left the entry.
+                    const unsigned long long free_for_me =
+                        ((unsigned long long)(block - RING) << 32) | hz::BARRIER_NO_LINK;
+                    if (seen == free_for_me) {
+                        pending = false;
+                        unsigned int fresh = arena_alloc_any();
+                        if (fresh == hz::LINK_NONE) {
+                            set_flags(ARENA_EXHAUSTED_FLAG);
                         } else {
-                            stall(hz::STALL_BARRIER_INSTALL, cur);
+                            hz::wave::atomic_store(&_dev_ctx.link_aux[2u * fresh + 1u], 64u << 16);
+                            write_arena_slot(fresh, 0u, words);
+                            // The count word and slot 0 are written before the
+                            // block's other lanes read the link from the entry.
+                            hz::wave::atomic_store_release(entry, ((unsigned long long)block << 32) | fresh);
+                            barrier_finish(b, fresh, D, block);
+                            held = true;
                         }
                     }
+                } else if ((unsigned int)(seen >> 32) == block && (unsigned int)seen != hz::BARRIER_NO_LINK) {
+                    barrier_write(b, (unsigned int)seen, slot, D, words, block);
+                    pending = false;
+                    held = true;
                 }
-                // Otherwise another lane is installing a link: retry.
             }
             if (hz::wave::ballot(pending) != 0ull) hz::wave::sleep_short();
         }
@@ -1632,15 +1608,35 @@ class AsyncProgram {
     template <typename BARRIER>
     __device__ HZ_INLINE void barrier_release(BARRIER& b) {
         constexpr unsigned int NOPS = BARRIER::TYPE_COUNT;
+        constexpr unsigned int RING = BARRIER::RING;
         for (unsigned int d = 0; d < NOPS; ++d) {
-            unsigned long long old =
-                atomicExch(&b.partial_table[d], (unsigned long long)hz::BARRIER_RELEASED << 32);
-            unsigned int L = (unsigned int)(old >> 32);
-            unsigned int s = (unsigned int)old;
-            if (L >= _dev_ctx.links || s >= 64u) continue;  // none, or full: its writers hand it on
-            // Close the partial link at its reserved count.
-            unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], (s - 64u) << 16);
-            if ((before & 0xFFFFu) == s) barrier_complete(b, L, d, s);
+            // Close the ticket counter; the tickets handed out fill whole
+            // blocks and, unless T is a multiple of 64, part of block T / 64.
+            const unsigned int old = hz::wave::atomic_or(&b.tickets[d], BARRIER::RELEASED_BIT);
+            if (old & BARRIER::RELEASED_BIT) continue;
+            const unsigned int block = old / 64u, part = old % 64u;
+            if (part == 0u) continue;
+            // Wait for the last block's link (its slot-0 lane holds a ticket
+            // below T, so it installs it), then lower its target to `part`.
+            unsigned long long* entry = &b.ring[d][block % RING];
+            unsigned long long seen = 0ull;
+            bool found = false;
+            for (uint32_t spins = 0; spins < SLOT_SPIN_LIMIT; ++spins) {
+                seen = hz::wave::atomic_load_acquire(entry);
+                if ((unsigned int)(seen >> 32) == block && (unsigned int)seen != hz::BARRIER_NO_LINK) {
+                    found = true;
+                    break;
+                }
+                hz::wave::sleep_short();
+            }
+            if (!found) {
+                stall(hz::STALL_BARRIER_INSTALL, seen);
+                continue;
+            }
+            const unsigned int L = (unsigned int)seen;
+            const unsigned int before =
+                hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], (part - 64u) << 16);
+            if ((before & 0xFFFFu) == part) barrier_complete(b, L, d, part, block);
         }
         unsigned int h = atomicExch(&b.full_head, hz::BARRIER_RELEASED);
         hz::wave::fence_acquire();
@@ -1683,25 +1679,30 @@ class AsyncProgram {
     }
 
     template <typename BARRIER>
-    __device__ HZ_INLINE void barrier_write(BARRIER& b, unsigned int L, unsigned int slot, OpDisc d, const uint32_t* words) {
+    __device__ HZ_INLINE void barrier_write(BARRIER& b, unsigned int L, unsigned int slot, OpDisc d,
+                                            const uint32_t* words, unsigned int block) {
         write_arena_slot(L, slot, words);
-        barrier_finish(b, L, d);
+        barrier_finish(b, L, d, block);
     }
 
     // Counts one written slot; the lane that completes the link hands it on.
     template <typename BARRIER>
-    __device__ HZ_INLINE void barrier_finish(BARRIER& b, unsigned int L, OpDisc d) {
+    __device__ HZ_INLINE void barrier_finish(BARRIER& b, unsigned int L, OpDisc d, unsigned int block) {
         hz::wave::fence_release();
         unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], 1u);
         unsigned int filled = (before & 0xFFFFu) + 1u;
         unsigned int target = before >> 16;
-        if (filled == target) barrier_complete(b, L, d, target);
+        if (filled == target) barrier_complete(b, L, d, target, block);
     }
 
-    // A completed link goes onto the barrier's full list, or to the work
-    // queues once the barrier is released.
+    // A completed link leaves its ring entry to the block RING after it, and
+    // goes onto the barrier's full list, or to the work queues once the
+    // barrier is released.
     template <typename BARRIER>
-    __device__ HZ_INLINE void barrier_complete(BARRIER& b, unsigned int L, OpDisc d, unsigned int count) {
+    __device__ HZ_INLINE void barrier_complete(BARRIER& b, unsigned int L, OpDisc d, unsigned int count,
+                                               unsigned int block) {
+        hz::wave::atomic_store(&b.ring[d][block % BARRIER::RING],
+                               ((unsigned long long)block << 32) | hz::BARRIER_NO_LINK);
         _dev_ctx.link_meta[L] = d | (count << 16);
         hz::wave::fence_release();
         unsigned int h = hz::wave::atomic_load(&b.full_head);

@@ -10,15 +10,22 @@
 // host with blank()). Its methods delegate the holding and releasing to the
 // program type, so the same specification runs on the GPU runtime and on the
 // host executors:
-//  - AsyncProgram keeps held promises in arena links: a lane reserves a slot
-//    in its operation's partial link with one 64-bit atomic on
-//    partial_table[op] = (link << 32) | reserved, writes its promise into
-//    the link's structure-of-arrays columns, and counts itself in the link's
-//    (target << 16) | filled word; whoever brings `filled` to `target` hands
-//    the link on: onto the barrier's full list (a lock-free stack in
-//    full_head) or, once released, straight to the work queues. Release
-//    marks every partial word and the list head as released, closes partial
-//    links at their reserved count and moves the full list to the queues.
+//  - AsyncProgram keeps held promises in arena links, filled by tickets: an
+//    awaiting lane draws ticket t from its operation's counter tickets[op]
+//    with one atomic add; ticket t is slot t % 64 of block t / 64, and each
+//    block fills one arena link. The lane that draws a block's slot 0
+//    installs the block's link and publishes it in ring[op][block % RING];
+//    the other lanes of the block wait for that entry (reading only), write
+//    their promise into their slot and count themselves in the link's
+//    (target << 16) | filled word. Whoever brings `filled` to `target` marks
+//    the ring entry done (so block + RING may take it) and hands the link
+//    on: onto the barrier's full list (a lock-free stack in full_head) or,
+//    once released, straight to the work queues. Release sets the RELEASED
+//    bit of every ticket counter (later awaits run their promise at once),
+//    closes the last block of each operation at the tickets it handed out,
+//    and moves the full list to the queues. Every await costs one atomic on
+//    the counter however many lanes wait on the barrier, and no lane retries
+//    a reservation.
 //  - the host executors hold the promises in a side table.
 //
 // A semaphore decrement is a release and the decrement that reaches zero an
@@ -37,10 +44,26 @@ struct RemappingBarrier {
     using UnionType = PromiseUnion<typename hz::ExecOpSet<OP_SET>::type>;
     static const size_t TYPE_COUNT = UnionType::Info::COUNT;
 
+    // Blocks of 64 tickets per operation whose links may be in flight at once.
+    // A block's slot-0 lane waits for the block RING before it to fill, so
+    // RING x 64 should cover the awaits a device can have outstanding on one
+    // barrier at once (4 KiB of ring per operation; 64 barriers with 32,768
+    // awaits each took 362, 60 and 40 ms with rings of 8, 128 and 512).
+#ifndef HZ_BARRIER_RING
+#define HZ_BARRIER_RING 512
+#endif
+    static constexpr unsigned int RING = HZ_BARRIER_RING;
+    // Set in a ticket counter once the barrier is released.
+    static constexpr unsigned int RELEASED_BIT = 0x80000000u;
+
     TaggedSemaphore semaphore;
     unsigned int count;      // promises awaited on this barrier (statistic)
     unsigned int full_head;  // list of full held links, or BARRIER_RELEASED
-    unsigned long long int partial_table[TYPE_COUNT > 0 ? TYPE_COUNT : 1];
+    // Tickets handed out per operation (at most 2^31 before release).
+    unsigned int tickets[TYPE_COUNT > 0 ? TYPE_COUNT : 1];
+    // ring[op][block % RING] = (block << 32) | link: the link of a block
+    // being filled; (block << 32) | BARRIER_NO_LINK once it is full.
+    unsigned long long int ring[TYPE_COUNT > 0 ? TYPE_COUNT : 1][RING];
 
     RemappingBarrier() = default;
 
@@ -52,7 +75,11 @@ struct RemappingBarrier {
         b.count = 0u;
         b.full_head = hz::BARRIER_NO_LINK;
         for (size_t i = 0; i < (TYPE_COUNT > 0 ? TYPE_COUNT : 1); ++i) {
-            b.partial_table[i] = (unsigned long long)hz::BARRIER_NO_LINK << 32;
+            b.tickets[i] = 0u;
+            // Entry k starts as the finished block k - RING, so block k may install.
+            for (unsigned int k = 0; k < RING; ++k) {
+                b.ring[i][k] = ((unsigned long long)(unsigned int)(k - RING) << 32) | hz::BARRIER_NO_LINK;
+            }
         }
         return b;
     }

@@ -159,6 +159,16 @@ typedef struct hz_barrier_state {
  *       1 when finished, 0 when not, -1 on error.
  *   void clear_flags_<suffix>(void* instance);                (:170-177)
  *   void set_device_<suffix>(int device);                     (:179-184)
+ *
+ * Additions with no reference counterpart (the reference launches on the
+ * null stream and synchronizes after every launch):
+ *   void init_program_on_<suffix>(void* instance, size_t grid, void* stream);
+ *   void exec_program_on_<suffix>(void* instance, size_t grid, size_t cycles,
+ *                                 void* stream);
+ *       Enqueue the same work on a hipStream_t and return without waiting,
+ *       so that instances on different streams overlap. The device state is
+ *       read at init_program_on; the caller synchronizes the stream before
+ *       complete_<suffix>.
  */
 #define HZ_DECLARE_PROGRAM_ABI(suffix)                                              \
     void* alloc_state_##suffix(void);                                               \
@@ -171,7 +181,9 @@ typedef struct hz_barrier_state {
     void exec_program_##suffix(void* instance_ptr, size_t grid_size, size_t cycle_count); \
     int complete_##suffix(void* instance_ptr);                                      \
     void clear_flags_##suffix(void* instance_ptr);                                  \
-    void set_device_##suffix(int device);
+    void set_device_##suffix(int device);                                           \
+    void init_program_on_##suffix(void* instance_ptr, size_t grid_size, void* stream); \
+    void exec_program_on_##suffix(void* instance_ptr, size_t grid_size, size_t cycle_count, void* stream);
 
 HZ_DECLARE_PROGRAM_ABI(collatz_event)
 HZ_DECLARE_PROGRAM_ABI(collatz_async)
@@ -255,7 +267,11 @@ long long wall_clock_rate(void);
  * hz_exec launches once and synchronizes (the reference's convention);
  * hz_complete sets *done to 1 when the program has finished. The state lives
  * on the device; hz_state_load/store copy a byte range of it and take effect
- * from the next launch. */
+ * from the next launch.
+ * hz_init_on / hz_exec_on enqueue on `stream` (a hipStream_t; NULL = the
+ * null stream) and return at once; hz_complete_on synchronizes that stream
+ * and then reports like hz_complete. hz_stream_create/destroy make and free
+ * non-blocking streams for callers without HIP headers. */
 #define HZ_SPEC_COLLATZ 0
 #define HZ_SPEC_PHOTON 1
 #define HZ_SPEC_BFS 2
@@ -268,6 +284,11 @@ int hz_init(hz_instance* instance, uint32_t groups);
 int hz_exec(hz_instance* instance, uint32_t groups, uint32_t cycles);
 int hz_complete(hz_instance* instance, int* done);
 int hz_clear_flags(hz_instance* instance);
+int hz_init_on(hz_instance* instance, uint32_t groups, void* stream);
+int hz_exec_on(hz_instance* instance, uint32_t groups, uint32_t cycles, void* stream);
+int hz_complete_on(hz_instance* instance, void* stream, int* done);
+int hz_stream_create(void** stream);
+int hz_stream_destroy(void* stream);
 int hz_state_load(hz_instance* instance, uint64_t offset, uint64_t bytes, void* host);
 int hz_state_store(hz_instance* instance, uint64_t offset, uint64_t bytes, const void* host);
 int hz_set_device(int device);

@@ -15,7 +15,7 @@ HEADER = ROOT / "include" / "harmonize_amd.h"
 PROGRAM_ENTRY_POINTS = [
     "alloc_state_{}", "free_state_{}", "load_state_device_{}", "store_state_device_{}",
     "alloc_program_{}", "free_program_{}", "init_program_{}", "exec_program_{}",
-    "complete_{}", "clear_flags_{}", "set_device_{}",
+    "complete_{}", "clear_flags_{}", "set_device_{}", "init_program_on_{}", "exec_program_on_{}",
 ]
 
 

@@ -56,6 +56,45 @@ def test_collatz_through_instance(kind, size, cycles):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind,size,cycles", [("async", 1 << 12, 64), ("event", 1 << 18, 4)])
+def test_two_instances_on_their_own_streams(kind, size, cycles):
+    # Launches of two instances are enqueued on two streams without waiting,
+    # alternating, until both report completion; each matches the oracle.
+    import harmonize_amd as hz
+
+    runs = []
+    streams = [hz.stream_create(), hz.stream_create()]
+    try:
+        for seed, n in ((6, 30000), (7, 12345)):
+            host = O.collatz_make_starts(seed, 40, n)
+            starts = hz.DeviceArray.from_numpy(host)
+            steps = hz.DeviceArray(n, np.uint32)
+            it = hz.DeviceArray.from_numpy(np.array([0, n], np.uint32))
+            st = hz.CollatzState(starts.ptr, steps.ptr, it.ptr)
+            runs.append([host, starts, steps, it, hz.Instance(kind, hz.SPEC_COLLATZ, size, st), False])
+        for r, s in zip(runs, streams):
+            r[4].init_on(128, s)
+        for _ in range(100000):
+            for r, s in zip(runs, streams):
+                if not r[5]:
+                    r[4].exec_on(128, cycles, s)
+            for r, s in zip(runs, streams):
+                if not r[5]:
+                    r[5] = r[4].complete_on(s)
+            if all(r[5] for r in runs):
+                break
+        else:
+            raise AssertionError("the programs did not complete")
+        for host, _, steps, _, inst, _ in runs:
+            want, _ = O.collatz_steps(host)
+            assert np.array_equal(steps.numpy(), want)
+            inst.close()
+    finally:
+        for s in streams:
+            hz.stream_destroy(s)
+
+
+@pytest.mark.gpu
 def test_serve_poll_reports_each_job_once():
     import harmonize_amd as hz
 
