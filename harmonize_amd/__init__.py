@@ -104,6 +104,10 @@ _SIGNATURES = {
     ),
     "hz_collatz_make_starts": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _VP]),
     "hz_audit_selftest": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "hz_tune_cache_set_path": (ctypes.c_int, [ctypes.c_char_p]),
+    "hz_tune_cache_lookup": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(RunConfig),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "hz_tune_cache_store": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(RunConfig), ctypes.c_double]),
     "hz_chase_run": (
         ctypes.c_int,
         [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _VP, _VP, _VP, _VP, ctypes.POINTER(RunConfig),
@@ -549,6 +553,8 @@ class TuneResult(ctypes.Structure):
         ("start_ms", ctypes.c_double),
         ("trials", ctypes.c_uint32),
         ("failures", ctypes.c_uint32),
+        ("cached", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -577,8 +583,27 @@ def _tune_out(res: TuneResult, log, n: int) -> dict:
         "start_ms": res.start_ms,
         "trials": res.trials,
         "failures": res.failures,
+        "cached": bool(res.cached),
         "log": [(_config_dict(log[i].config), log[i].ms) for i in range(min(n, res.trials))],
     }
+
+
+def tune_cache_set_path(path: str | None) -> None:
+    """Sets the tuning cache file (None disables it); see hz_tune_cache_set_path."""
+    check(lib().hz_tune_cache_set_path(path.encode() if path else None), "hz_tune_cache_set_path")
+
+
+def tune_cache_lookup(key: str):
+    """(config dict, best_ms) stored under `key`, or None."""
+    c, ms, found = RunConfig(), ctypes.c_double(), ctypes.c_int()
+    check(lib().hz_tune_cache_lookup(key.encode(), ctypes.byref(c), ctypes.byref(ms), ctypes.byref(found)),
+          "hz_tune_cache_lookup")
+    return (_config_dict(c), ms.value) if found.value else None
+
+
+def tune_cache_store(key: str, config: dict, best_ms: float) -> None:
+    check(lib().hz_tune_cache_store(key.encode(), ctypes.byref(make_config(**config)), best_ms),
+          "hz_tune_cache_store")
 
 
 def tune_collatz(steps: DeviceArray, starts: DeviceArray | None, count: int, rounds: int = 2, reps: int = 1) -> dict:

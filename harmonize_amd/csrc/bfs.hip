@@ -321,11 +321,7 @@ std::unique_ptr<Runner> make_async_runner(hz_bfs_batch& b, const hz_run_config& 
     switch (c.stash) {
         case 0:
         case 16:
-#ifdef HZ_BFS_STASH_ALT
-            return make_async_runner_stash<HZ_BFS_STASH_ALT>(b, c);
-#else
             return make_async_runner_stash<16>(b, c);
-#endif
         case 8:
             return make_async_runner_stash<8>(b, c);
         default:

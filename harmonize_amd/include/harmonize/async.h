@@ -1186,9 +1186,6 @@ class AsyncProgram {
             }
             hz::wave::sync();
         }
-#ifndef HZ_DROP_AFTER_BURST
-        drop_holding_if_empty();
-#endif
     }
 
     // True if the group's home queue holds links (at some depth level).
@@ -1334,9 +1331,9 @@ class AsyncProgram {
                 if ((uint32_t)__builtin_popcount(fm2) < MARGIN) break;
                 run_link(pick_ready(rm2));
             }
-#ifdef HZ_DROP_AFTER_BURST
+            // A group whose stash emptied stops counting as holding work once
+            // per cycle, not after every link (Collatz 66.8 -> 64.5 ms).
             drop_holding_if_empty();
-#endif
             return RUN;
         }
 

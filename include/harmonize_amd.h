@@ -503,13 +503,25 @@ int hz_bfs_batch_release(hz_bfs_batch* batch);
  * the fastest) for every program; BFS adds block_groups, stash and the edge
  * split threshold and tunes the hipGraph-launched pass. The workload's
  * outputs hold the last trial's results. `log` (may be NULL) receives up to
- * log_capacity trials in order. */
+ * log_capacity trials in order.
+ *
+ * Results are cached per program, device and workload class when a cache
+ * file is set (hz_tune_cache_set_path, or the HZ_TUNE_CACHE environment
+ * variable): a tuner call whose key is in the file returns the stored
+ * configuration without running a trial (trials = 0, cached = 1), and a
+ * tuning run stores its result. The key is
+ * "<program>|<gcn arch>|<compute units>|<workload class>", the class being
+ * the workload's size rounded to a power of two (Collatz starts, photons;
+ * BFS graphs, vertices and edges). The file holds one line per key:
+ * key, tab, the ten hz_run_config fields, tab, best_ms. */
 typedef struct hz_tune_result {
     hz_run_config best;
     double best_ms;   /* time of the best configuration */
     double start_ms;  /* time of the starting configuration (-1: it failed) */
     uint32_t trials;  /* runs made */
     uint32_t failures;
+    uint32_t cached;  /* 1: taken from the cache, no trial run */
+    uint32_t reserved;
 } hz_tune_result;
 typedef struct hz_tune_trial {
     hz_run_config config;
@@ -522,6 +534,11 @@ int hz_photon_tune(const hz_photon_params* params, double* d_tally, uint32_t* d_
                    uint32_t reps, hz_tune_result* result, hz_tune_trial* log, uint32_t log_capacity);
 int hz_bfs_batch_tune(hz_bfs_batch* batch, uint32_t rounds, uint32_t reps, hz_tune_result* result,
                       hz_tune_trial* log, uint32_t log_capacity);
+/* Sets the tuning cache file (NULL or "" disables the cache). */
+int hz_tune_cache_set_path(const char* path);
+/* Looks up / stores one cache entry by key (host only). */
+int hz_tune_cache_lookup(const char* key, hz_run_config* config, double* best_ms, int* found);
+int hz_tune_cache_store(const char* key, const hz_run_config* config, double best_ms);
 
 /* ------------------------------------------------------------- serving
  * A request server for Collatz, photon-walk and BFS jobs. Up to `slots` units

@@ -160,3 +160,29 @@ def test_audit_catches_planted_inconsistencies():
     buf = ctypes.create_string_buffer(4096)
     missed = hz.lib().hz_audit_selftest(buf, len(buf))
     assert missed == 0, buf.value.decode()
+
+
+def test_tune_cache_round_trip(tmp_path):
+    # The tuning cache file (host only): entries round-trip, a second store
+    # under the same key replaces the first, other keys are untouched, and a
+    # disabled cache finds nothing.
+    path = tmp_path / "tune.txt"
+    hz.tune_cache_set_path(str(path))
+    try:
+        assert hz.tune_cache_lookup("bfs_batch_async|gfx950|256|graphs=64") is None
+        a = dict(groups=2048, arena=466741, queues=16, coalesce=32, block_groups=2, flags=1, spec_param=128, stash=8)
+        hz.tune_cache_store("bfs_batch_async|gfx950|256|graphs=64", a, 37.6)
+        hz.tune_cache_store("collatz_async|gfx950|256|starts=2^26", dict(queues=16), 67.0)
+        b = dict(a, spec_param=64)
+        hz.tune_cache_store("bfs_batch_async|gfx950|256|graphs=64", b, 36.9)
+        got, ms = hz.tune_cache_lookup("bfs_batch_async|gfx950|256|graphs=64")
+        assert ms == 36.9 and {k: v for k, v in got.items() if v} == b
+        got, ms = hz.tune_cache_lookup("collatz_async|gfx950|256|starts=2^26")
+        assert got["queues"] == 16 and ms == 67.0
+        assert len(path.read_text().splitlines()) == 2
+        with pytest.raises(hz.HarmonizeError):
+            hz.tune_cache_store("bad\tkey", a, 1.0)
+        hz.tune_cache_set_path(None)
+        assert hz.tune_cache_lookup("collatz_async|gfx950|256|starts=2^26") is None
+    finally:
+        hz.tune_cache_set_path(None)

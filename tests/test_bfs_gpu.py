@@ -175,3 +175,22 @@ def test_tuner_picks_a_working_configuration(hz, small_batch):
     assert np.array_equal(b.levels(), want)
     assert st["launches"] >= 1
     b.release()
+
+
+def test_tuner_cache_returns_the_stored_choice(hz, small_batch, tmp_path):
+    """A second tuning of the same batch class on the same device comes from the
+    cache: no trial, the same configuration, which still computes the levels."""
+    b, want = small_batch
+    hz.tune_cache_set_path(str(tmp_path / "tune.txt"))
+    try:
+        first = hz.tune_bfs(b, rounds=1, reps=1)
+        second = hz.tune_bfs(b, rounds=1, reps=1)
+        assert not first["cached"] and first["trials"] > 10
+        assert second["cached"] and second["trials"] == 0 and second["best"] == first["best"]
+        best = {k: v for k, v in second["best"].items() if v}
+        best.setdefault("max_launches", 1 << 16)
+        b.run("async", **best)
+        assert np.array_equal(b.levels(), want)
+    finally:
+        hz.tune_cache_set_path(None)
+        b.release()
