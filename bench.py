@@ -366,6 +366,66 @@ def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -
     return out
 
 
+def resident_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -> dict:
+    """Collatz jobs only (sizes log-uniform in [2^10, 2^22] starts below
+    2^32, per-job seeds), served two ways on the same trace: admitted into a
+    running resident kernel (hz_resident_*, arrival order and smallest job
+    first) and by the slot server (one launch per job or chunk on 3 streams).
+    Peak throughput with every job arriving at once, then Poisson arrivals at
+    fractions of each server's own peak; also one 2^22-start job alone."""
+    rng = np.random.default_rng(1)
+    sizes = np.rint(2.0 ** rng.uniform(10.0, 22.0, size=jobs)).astype(np.int64)
+    seeds = rng.integers(1, 1 << 40, size=jobs)
+    trace = list(zip(sizes.tolist(), seeds.tolist()))
+    unit = np.random.default_rng(2).exponential(1.0, size=jobs)
+    out = {"workload": f"{jobs} Collatz jobs, sizes log-uniform in [2^10, 2^22] starts uniform in [1, 2^32), "
+                       f"per-job seeds; the same trace for every server and load"}
+
+    def sweep(srv, run):
+        run(srv, [(1024, 5)] * 8, np.zeros(8))
+        lone = [run(srv, [(1 << 22, 77 + k)], np.zeros(1))[0] for k in range(3)]
+        res = run(srv, trace, np.zeros(jobs))
+        span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
+        peak = jobs / (span * 1e-3)
+        entry = {"lone_2^22_ms": float(np.median([r["end_ms"] - r["submit_ms"] for r in lone])),
+                 "peak_jobs_per_s": peak, "peak_makespan_ms": span, "loads": []}
+        for load in loads:
+            arrivals = np.cumsum(unit / (load * peak)) * 1e3
+            arrivals -= arrivals[0]
+            res = run(srv, trace, arrivals)
+            lat = np.array([r["end_ms"] - r["submit_ms"] for r in res])
+            span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
+            entry["loads"].append({"offered_fraction_of_peak": load, "offered_jobs_per_s": load * peak,
+                                   "achieved_jobs_per_s": jobs / (span * 1e-3),
+                                   "p50_ms": float(np.percentile(lat, 50)),
+                                   "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())})
+        return entry
+
+    def run_resident(srv, tr, arrivals):
+        return srv.run_trace(tr, arrivals)
+
+    def run_slots(srv, tr, arrivals):
+        return srv.run_trace([("collatz", s, sd) for s, sd in tr], arrivals)
+
+    for name, make, run in (
+        ("resident_arrival_order", lambda: hz.Resident(pool_bytes=24 << 30), run_resident),
+        ("resident_smallest_first", lambda: hz.Resident(pool_bytes=24 << 30, policy=hz.RESIDENT_SMALLEST_FIRST),
+         run_resident),
+        ("slots", lambda: hz.Server(pool_bytes=24 << 30, slots=3), run_slots),
+    ):
+        srv = make()
+        try:
+            out[name] = sweep(srv, run)
+            if name.startswith("resident"):
+                info = srv.info()
+                out[name]["launches"] = info["launches"]
+                out[name]["jobs_joined_running_launch"] = info["joined_running"]
+                out[name]["jobs"] = info["jobs"]
+        finally:
+            srv.close()
+    return out
+
+
 def _leg_collatz(kind):
     def run(hz):
         starts = hz.collatz_make_starts(SEED, BITS, COUNT)
@@ -575,6 +635,7 @@ def main() -> int:
 
     if not args.no_serve:
         out["serving"] = serving_sweep(hz)
+        out["serving_resident"] = resident_sweep(hz)
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
     if not np.array_equal(steps_out.numpy(), steps_async):

@@ -801,6 +801,134 @@ class Server:
             pass
 
 
+# ------------------------------------------------------------ resident serving
+
+RESIDENT_BY_ADMISSION = 0
+RESIDENT_SMALLEST_FIRST = 1
+
+
+class ResidentConfig(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_uint64),
+        ("arena_links", ctypes.c_uint64),
+        ("groups", ctypes.c_uint32),
+        ("policy", ctypes.c_uint32),
+        ("idle_us", ctypes.c_uint32),
+        ("poll_us", ctypes.c_uint32),
+        ("life_ms", ctypes.c_uint32),
+        ("bits", ctypes.c_uint32),
+        ("keep_outputs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class ResidentInfo(ctypes.Structure):
+    _fields_ = [
+        ("jobs", ctypes.c_uint64),
+        ("pending", ctypes.c_uint32),
+        ("in_flight", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
+        ("joined_running", ctypes.c_uint32),
+        ("groups", ctypes.c_uint32),
+        ("running", ctypes.c_uint32),
+        ("pool_bytes", ctypes.c_uint64),
+        ("pool_used", ctypes.c_uint64),
+        ("promises", ctypes.c_uint64),
+        ("links_run", ctypes.c_uint64),
+        ("make_work_calls", ctypes.c_uint64),
+        ("idle_polls", ctypes.c_uint64),
+        ("phase_cycles", ctypes.c_uint64 * 8),
+    ]
+
+
+_SIGNATURES.update(
+    {
+        "hz_resident_create": (ctypes.c_int, [ctypes.POINTER(ResidentConfig), ctypes.POINTER(_VP)]),
+        "hz_resident_destroy": (ctypes.c_int, [_VP]),
+        "hz_resident_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)]),
+        "hz_resident_poll": (ctypes.c_int, [_VP, _VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+        "hz_resident_drain": (ctypes.c_int, [_VP, ctypes.c_double]),
+        "hz_resident_job": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(JobResult)]),
+        "hz_resident_output": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_uint32, _VP, ctypes.c_uint64]),
+        "hz_resident_release": (ctypes.c_int, [_VP, ctypes.c_uint64]),
+        "hz_resident_info": (ctypes.c_int, [_VP, ctypes.POINTER(ResidentInfo)]),
+        "hz_resident_run_trace": (ctypes.c_int, [_VP, ctypes.POINTER(Job), _VP, ctypes.c_uint32, ctypes.c_double,
+                                                 ctypes.POINTER(JobResult)]),
+    }
+)
+
+
+class Resident:
+    """Collatz jobs admitted into a running persistent kernel (hz_resident_*)."""
+
+    def __init__(self, pool_bytes: int = 0, arena_links: int = 0, groups: int = 0,
+                 policy: int = RESIDENT_BY_ADMISSION, idle_us: int = 0, poll_us: int = 0, life_ms: int = 0,
+                 bits: int = 0, keep_outputs: bool = False):
+        cfg = ResidentConfig(pool_bytes, arena_links, groups, policy, idle_us, poll_us, life_ms, bits,
+                             1 if keep_outputs else 0, 0)
+        h = _VP()
+        check(lib().hz_resident_create(ctypes.byref(cfg), ctypes.byref(h)), "hz_resident_create")
+        self.handle = h.value
+
+    def submit(self, size: int, seed: int) -> int:
+        i = ctypes.c_uint64()
+        j = _job("collatz", size, seed)
+        check(lib().hz_resident_submit(self.handle, ctypes.byref(j), ctypes.byref(i)), "hz_resident_submit")
+        return i.value
+
+    def poll(self, capacity: int = 64) -> list:
+        ids = np.zeros(capacity, dtype=np.uint64)
+        n = ctypes.c_uint32()
+        check(lib().hz_resident_poll(self.handle, ids.ctypes.data, capacity, ctypes.byref(n)), "hz_resident_poll")
+        return [int(x) for x in ids[: n.value]]
+
+    def drain(self, timeout_ms: float = 600000.0) -> None:
+        check(lib().hz_resident_drain(self.handle, timeout_ms), "hz_resident_drain")
+
+    def job(self, i: int) -> dict:
+        r = JobResult()
+        check(lib().hz_resident_job(self.handle, i, ctypes.byref(r)), "hz_resident_job")
+        return r.as_dict()
+
+    def output(self, i: int, which: int = 0) -> np.ndarray:
+        n = self.job(i)["size"]
+        out = np.empty(n, dtype=np.uint32 if which == 0 else np.uint64)
+        check(lib().hz_resident_output(self.handle, i, which, out.ctypes.data, out.nbytes), "hz_resident_output")
+        return out
+
+    def release(self, i: int) -> None:
+        check(lib().hz_resident_release(self.handle, i), "hz_resident_release")
+
+    def info(self) -> dict:
+        s = ResidentInfo()
+        check(lib().hz_resident_info(self.handle, ctypes.byref(s)), "hz_resident_info")
+        out = {name: getattr(s, name) for name, _ in s._fields_}
+        out["phase_cycles"] = list(s.phase_cycles)
+        return out
+
+    def run_trace(self, sizes_seeds, arrival_ms, timeout_ms: float = 600000.0) -> list:
+        """Replays (size, seed) Collatz jobs arriving at arrival_ms (sorted, ms
+        after the call); returns one result dict per job."""
+        n = len(sizes_seeds)
+        arr = (Job * n)(*[_job("collatz", s, sd) for s, sd in sizes_seeds])
+        t = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        res = (JobResult * n)()
+        check(lib().hz_resident_run_trace(self.handle, arr, t.ctypes.data, n, timeout_ms, res),
+              "hz_resident_run_trace")
+        return [r.as_dict() for r in res]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().hz_resident_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ------------------------------------------------------------ generic instances
 
 SPEC_COLLATZ, SPEC_PHOTON, SPEC_BFS, SPEC_BARRIER = 0, 1, 2, 3

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pool.h"
 #include "collatz.h"
 #include "collatz_kernels.h"
 #include "graph.h"
@@ -36,6 +37,7 @@
 
 using hz::capi::check;
 using hz::capi::guarded;
+using hz::capi::Pool;
 
 namespace {
 
@@ -60,69 +62,6 @@ constexpr uint64_t BFS_EVENT_IO = 1ull << 23;    // per-operation queue capacity
 constexpr uint32_t BFS_GENERATIONS_PER_PASS = 4; // generations enqueued between completion checks
 
 uint64_t align_up(uint64_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
-
-// First-fit allocator over [0, size) with coalescing of free neighbours.
-class Pool {
-   public:
-    explicit Pool(uint64_t size) : size_(size) {
-        if (size) free_[0] = size;
-    }
-    static constexpr uint64_t NONE = ~0ull;
-
-    uint64_t alloc(uint64_t bytes) {
-        bytes = align_up(bytes ? bytes : 1);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                uint64_t off = it->first, len = it->second;
-                free_.erase(it);
-                if (len > bytes) free_[off + bytes] = len - bytes;
-                used_ += bytes;
-                return off;
-            }
-        }
-        return NONE;
-    }
-
-    void free(uint64_t off, uint64_t bytes) {
-        bytes = align_up(bytes ? bytes : 1);
-        if (off + bytes > size_) throw std::invalid_argument("pool: free outside the pool");
-        auto next = free_.lower_bound(off);
-        if (next != free_.end() && next->first < off + bytes) throw std::invalid_argument("pool: double free");
-        if (next != free_.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second > off) throw std::invalid_argument("pool: double free");
-        }
-        used_ -= bytes;
-        uint64_t start = off, len = bytes;
-        if (next != free_.end() && next->first == off + bytes) {
-            len += next->second;
-            next = free_.erase(next);
-        }
-        if (next != free_.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second == start) {
-                start = prev->first;
-                len += prev->second;
-                free_.erase(prev);
-            }
-        }
-        free_[start] = len;
-    }
-
-    uint64_t largest_free() const {
-        uint64_t best = 0;
-        for (auto& kv : free_) best = kv.second > best ? kv.second : best;
-        return best;
-    }
-    uint64_t used() const { return used_; }
-    uint64_t size() const { return size_; }
-    size_t fragments() const { return free_.size(); }
-
-   private:
-    uint64_t size_;
-    uint64_t used_ = 0;
-    std::map<uint64_t, uint64_t> free_;
-};
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -403,6 +403,7 @@ class AsyncProgram {
         unsigned long long int idle;
         unsigned long long int runs;
         uint32_t idle_streak;
+        uint32_t make_pause;   // cycles before make_work is called again (pause_make_work)
         uint32_t queue_check;  // cycles until the device queues are consulted again
         uint32_t share_check;  // cycles until the hungry count is consulted again
         uint32_t hungry;       // this group is counted in status->hungry
@@ -974,6 +975,16 @@ class AsyncProgram {
         return hz::wave::first(_grp_ctx.level);
     }
 
+    // Called from make_work (by the whole wavefront) when it found nothing
+    // to start but more may come later (a resident program waiting for
+    // jobs): the group skips make_work for the next `cycles` scheduling
+    // cycles, running its own links meanwhile, and pauses between cycles if
+    // it has none. It stays a maker, so the launch does not end.
+    __device__ HZ_INLINE void pause_make_work(unsigned int cycles) {
+        if (hz::wave::lane() == 0) _grp_ctx.make_pause = cycles;
+        hz::wave::sync();
+    }
+
    protected:
 
     template <typename TYPE>
@@ -1037,22 +1048,26 @@ class AsyncProgram {
         }
     }
 
+    // The group joins `units` only if make_work left promises in its stash,
+    // so a call that made nothing (an idle resident program polling for
+    // jobs) costs no atomic. While make_work runs the group is still a maker,
+    // which blocks termination by itself; the units increment precedes the
+    // release that gives up the maker role.
     __device__ HZ_INLINE void make_work_step() {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
-        if (lane == 0) {
-            if (!g.holding) {
-                g.holding = 1;
-                add_units(1);
-            }
-            g.made += 1;
-        }
+        if (lane == 0) g.made += 1;
         hz::wave::sync();
         if constexpr (LEVELS > 1) g.level = 0u;
         bool mine;
         [[clang::always_inline]] mine = PROGRAM_SPEC::make_work(*this);
         bool more = hz::wave::ballot(mine) != 0;
+        const bool holds = !stash_empty();
         if (lane == 0) {
+            if (holds && !g.holding) {
+                g.holding = 1;
+                add_units(1);
+            }
             g.can_make_work = more ? 1u : 0u;
             if (!more && g.maker) {
                 g.maker = 0;
@@ -1274,6 +1289,11 @@ class AsyncProgram {
         }
 
         uint32_t rm = hz::wave::first(g.ready_mask);
+        const uint32_t pause = hz::wave::first(g.make_pause);
+        if (pause != 0u) {
+            if (hz::wave::lane() == 0) g.make_pause = pause - 1u;
+            hz::wave::sync();
+        }
         if (rm == 0) {
             bool check = !hz::wave::first(g.can_make_work);
             if (!check) {
@@ -1293,7 +1313,9 @@ class AsyncProgram {
             }
             // Make work while no link is full, bounded so open links still run.
             HZ_PHASE(PHASE_MAKE);
-            for (uint32_t tries = 0; rm == 0 && tries < 4 && hz::wave::first(g.can_make_work); ++tries) {
+            for (uint32_t tries = 0; rm == 0 && tries < 4 && pause == 0u && hz::wave::first(g.can_make_work) &&
+                                     hz::wave::first(g.make_pause) == 0u;
+                 ++tries) {
                 uint32_t c = 0;
                 fullest_open(c);
                 if (tries > 0 && c >= _dev_ctx.coalesce) break;
@@ -1340,7 +1362,10 @@ class AsyncProgram {
         // Nothing local: look for queued links anywhere, then test for quiescence.
         HZ_PHASE(PHASE_IDLE);
         if (pull_links(1) > 0) return RUN;
-        if (hz::wave::first(g.can_make_work)) return RUN;
+        if (hz::wave::first(g.can_make_work)) {
+            if (pause != 0u) __builtin_amdgcn_s_sleep(8);
+            return RUN;
+        }
         set_hungry(true);
         uint32_t streak = hz::wave::first(g.idle_streak);
         if (streak >= IDLE_EXIT_POLLS) return DONE;
@@ -1393,6 +1418,7 @@ class AsyncProgram {
             g.home = blockIdx.x % _dev_ctx.queue_count;
             g.executed = g.spills = g.fills = g.made = g.idle = g.runs = 0;
             g.idle_streak = 0;
+            g.make_pause = 0;
             g.queue_check = 0;
             g.share_check = 0;
             g.hungry = 0;

@@ -641,6 +641,67 @@ int hz_server_run_trace(hz_server* server, const hz_job* jobs, const double* arr
  * (done or failed) since the previous poll. */
 int hz_serve_submit(hz_server* server, const hz_job* job, uint64_t* id);
 int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, uint32_t* count);
+/* Resident serving: Collatz jobs admitted into a running persistent kernel.
+ * The serving plan the reference's design calls for (a resident kernel fed by
+ * a host doorbell; the reference itself has no serving loop): the host writes
+ * each job's descriptor into a ring in host-coherent memory, a running
+ * AsyncProgram launch installs it into its job table and claims the job's
+ * items in priority order, and reports the job's completion in the ring; a
+ * launch with no job for idle_us ends, and the next submission launches
+ * again. Jobs are HZ_JOB_COLLATZ with starts uniform in [1, 2^bits) from the
+ * seed (the serving loop's generator); outputs: 0 = step counts (u32),
+ * 1 = starts (u64). At most 64 jobs are in flight; the rest wait in order. */
+typedef struct hz_resident hz_resident;
+typedef struct hz_resident_config {
+    uint64_t pool_bytes;   /* device memory for job outputs (0 = 4 GiB) */
+    uint64_t arena_links;  /* link arena of the program (0 = 2^18) */
+    uint32_t groups;       /* groups of a launch (0 = every group that fits on the device) */
+    uint32_t policy;       /* claim order: HZ_RESIDENT_BY_ADMISSION (0) or HZ_RESIDENT_SMALLEST_FIRST */
+    uint32_t idle_us;      /* a launch with no job this long ends (0 = 1000) */
+    uint32_t poll_us;      /* the kernel's housekeeping period (0 = 2) */
+    uint32_t life_ms;      /* a launch stops claiming after this long; the next one continues (0 = 10000) */
+    uint32_t bits;         /* Collatz starts in [1, 2^bits), 1..40 (0 = 32) */
+    uint32_t keep_outputs; /* 1: a finished job keeps its outputs until hz_resident_release */
+    uint32_t reserved;
+} hz_resident_config;
+#define HZ_RESIDENT_BY_ADMISSION 0u
+#define HZ_RESIDENT_SMALLEST_FIRST 1u
+typedef struct hz_resident_info_t {
+    uint64_t jobs;
+    uint32_t pending;        /* waiting for a job-table entry or pool memory */
+    uint32_t in_flight;      /* published and not finished */
+    uint32_t launches;       /* kernel launches so far */
+    uint32_t joined_running; /* jobs published while a launch was running */
+    uint32_t groups;
+    uint32_t running;
+    uint64_t pool_bytes;
+    uint64_t pool_used;
+    /* Runtime counters summed over the launches that have ended. */
+    uint64_t promises;
+    uint64_t links_run;
+    uint64_t make_work_calls;
+    uint64_t idle_polls;
+    uint64_t phase_cycles[8]; /* filled only in a build with HZ_PHASE_TIMING */
+} hz_resident_info_t;
+int hz_resident_create(const hz_resident_config* config, hz_resident** out);
+/* Waits for the running launch to finish its jobs and end, then frees all. */
+int hz_resident_destroy(hz_resident* server);
+/* Queues a job and makes progress (publishes it if the table has room). */
+int hz_resident_submit(hz_resident* server, const hz_job* job, uint64_t* id);
+/* Makes progress and returns (up to `capacity`) the ids of jobs that
+ * finished since the previous poll. */
+int hz_resident_poll(hz_resident* server, uint64_t* finished_ids, uint32_t capacity, uint32_t* count);
+/* Makes progress until every job has finished and the launch has ended. */
+int hz_resident_drain(hz_resident* server, double timeout_ms);
+/* result->launches is 0 for a job published while a launch was running. */
+int hz_resident_job(hz_resident* server, uint64_t id, hz_job_result* result);
+int hz_resident_output(hz_resident* server, uint64_t id, uint32_t which, void* host, uint64_t bytes);
+int hz_resident_release(hz_resident* server, uint64_t id);
+int hz_resident_info(const hz_resident* server, hz_resident_info_t* info);
+/* Open-loop replay as hz_server_run_trace. */
+int hz_resident_run_trace(hz_resident* server, const hz_job* jobs, const double* arrival_ms, uint32_t count,
+                          double timeout_ms, hz_job_result* results);
+
 /* Host-only check of the job-state pool allocator: ops[i] >= 0 allocates
  * that many bytes (out[i] = offset, or all-ones if it does not fit);
  * ops[i] = -k - 1 frees the k-th allocation (out[i] = largest free block). */

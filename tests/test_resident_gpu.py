@@ -1,0 +1,134 @@
+"""Resident serving on the GPU: Collatz jobs admitted into a running
+persistent kernel through the doorbell ring. Every job's starts and step
+counts against the oracle; more jobs than the job table holds, and more than
+the ring holds, published while one launch runs; the smallest-first claim
+order; a launch that ends when idle and the next submission launching again;
+launches cut short by their lifetime, carried on by the next launch."""
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+BITS = 32
+DRAIN_MS = 60000.0
+
+
+@pytest.fixture(scope="module")
+def hz():
+    import harmonize_amd
+
+    return harmonize_amd
+
+
+def check_job(res, i, size, seed):
+    starts = O.collatz_make_starts(seed, BITS, size)
+    assert np.array_equal(res.output(i, 1), starts), (i, size, seed)
+    want, _ = O.collatz_steps(starts)
+    assert np.array_equal(res.output(i, 0), want), (i, size, seed)
+
+
+@pytest.mark.parametrize("size", [1, 5000, (1 << 20) + 7, 1 << 22])
+def test_job_matches_oracle(hz, size):
+    res = hz.Resident(keep_outputs=True)
+    try:
+        seed = 0xA11CE + size
+        i = res.submit(size, seed)
+        res.drain(DRAIN_MS)
+        assert res.job(i)["status"] == hz.JOB_DONE
+        check_job(res, i, size, seed)
+        info = res.info()
+        assert info["running"] == 0 and info["in_flight"] == 0 and info["launches"] >= 1
+    finally:
+        res.close()
+
+
+def test_many_jobs_join_one_launch(hz):
+    """300 jobs submitted back to back: more than the 64-entry job table and
+    the 256-entry ring, so entries and ring slots are reused while the launch
+    runs; each job's outputs are its own."""
+    rng = np.random.default_rng(7)
+    sizes = np.rint(2.0 ** rng.uniform(4, 14, size=300)).astype(int)
+    seeds = rng.integers(1, 1 << 40, size=300)
+    res = hz.Resident(keep_outputs=True)
+    try:
+        ids = [res.submit(int(s), int(sd)) for s, sd in zip(sizes, seeds)]
+        res.drain(DRAIN_MS)
+        info = res.info()
+        assert info["joined_running"] > 0
+        assert info["launches"] < len(ids)
+        for i, s, sd in zip(ids, sizes, seeds):
+            assert res.job(i)["status"] == hz.JOB_DONE
+            check_job(res, i, int(s), int(sd))
+    finally:
+        res.close()
+
+
+def test_smallest_first_lets_small_jobs_overtake(hz):
+    """A 2^22-start job followed by small ones: claimed smallest first, every
+    small job finishes before the large one, and all outputs are right."""
+    res = hz.Resident(keep_outputs=True, policy=hz.RESIDENT_SMALLEST_FIRST)
+    try:
+        big = res.submit(1 << 22, 99)
+        small = [res.submit(1000 + 37 * k, 1000 + k) for k in range(20)]
+        res.drain(DRAIN_MS)
+        big_end = res.job(big)["end_ms"]
+        assert all(res.job(i)["end_ms"] < big_end for i in small)
+        check_job(res, big, 1 << 22, 99)
+        for k, i in enumerate(small):
+            check_job(res, i, 1000 + 37 * k, 1000 + k)
+    finally:
+        res.close()
+
+
+def test_idle_launch_ends_and_next_submission_relaunches(hz):
+    res = hz.Resident(keep_outputs=True, idle_us=200)
+    try:
+        a = res.submit(3000, 5)
+        res.drain(DRAIN_MS)
+        first = res.info()
+        assert first["running"] == 0
+        b = res.submit(4000, 6)
+        res.drain(DRAIN_MS)
+        second = res.info()
+        assert second["launches"] == first["launches"] + 1
+        check_job(res, a, 3000, 5)
+        check_job(res, b, 4000, 6)
+    finally:
+        res.close()
+
+
+def test_short_lifetime_launches_carry_on(hz):
+    """With a 1 ms launch lifetime a 2^22-start job spans several launches:
+    the job table and counters carry over, and the result is unchanged."""
+    res = hz.Resident(keep_outputs=True, life_ms=1)
+    try:
+        i = res.submit(1 << 22, 1234)
+        j = res.submit(777, 4321)
+        res.drain(DRAIN_MS)
+        assert res.info()["launches"] > 1
+        check_job(res, i, 1 << 22, 1234)
+        check_job(res, j, 777, 4321)
+    finally:
+        res.close()
+
+
+def test_open_loop_trace(hz):
+    """A Poisson trace of 40 jobs (sizes log-uniform in [2^10, 2^18]):
+    outputs equal the oracle's and latencies are measured from arrival."""
+    rng = np.random.default_rng(11)
+    sizes = np.rint(2.0 ** rng.uniform(10, 18, size=40)).astype(int)
+    seeds = rng.integers(1, 1 << 40, size=40)
+    arrivals = np.cumsum(rng.exponential(0.5, size=40))
+    res = hz.Resident(keep_outputs=True)
+    try:
+        out = res.run_trace(list(zip(sizes.tolist(), seeds.tolist())), arrivals)
+        for r, s, sd in zip(out, sizes, seeds):
+            assert r["status"] == hz.JOB_DONE
+            assert r["end_ms"] >= r["submit_ms"]
+            check_job(res, r["id"], int(s), int(sd))
+        res.drain(DRAIN_MS)
+    finally:
+        res.close()
