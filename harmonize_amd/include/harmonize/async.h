@@ -309,9 +309,13 @@ class AsyncProgram {
     static constexpr uint32_t MARGIN = OPS + 1u;
     // Ready links a group may run per scheduling cycle. Running up to four
     // took Collatz 2^26 from 70.7 to 67.0 ms and the photon walk from 5.78 to
-    // 5.66 ms; a program that shares work checks its hungry count once per
-    // cycle, so it runs one.
-    static constexpr uint32_t BURST = SHARE_WORK ? 1u : 4u;
+    // 5.66 ms, eight from 60.7 to 59.4 ms (with the group counters in
+    // registers); a program that shares work checks its hungry count once
+    // per cycle, so it runs one.
+#ifndef HZ_BURST
+#define HZ_BURST 8
+#endif
+    static constexpr uint32_t BURST = SHARE_WORK ? 1u : (uint32_t)HZ_BURST;
 
     // Asynchronous argument loads (reference ASYNC_LOADS, async/promise.h:
     // 284-303, program/async.h:2082-2144): an operation whose parameters
@@ -376,6 +380,18 @@ class AsyncProgram {
         // register: appends run in divergent code, where a shared counter
         // would need a cross-lane ordering on every call.
         unsigned long long appended;
+        // Promises evaluated and links run by this lane's group: the same
+        // value in every lane, kept in registers rather than read and
+        // written back in LDS on every link.
+        unsigned long long executed;
+        unsigned long long runs;
+        // Scheduler counters of the group, likewise uniform and in registers.
+        // They are written only where the whole wavefront is active.
+        unsigned long long made;   // make_work calls
+        unsigned long long idle;   // idle polls
+        uint32_t idle_streak;      // consecutive idle cycles
+        uint32_t queue_check;      // cycles until the device queues are consulted again
+        uint32_t make_pause;       // cycles before make_work is called again (pause_make_work)
     };
 
     // Per-wavefront scheduler state, in LDS.
@@ -396,15 +412,8 @@ class AsyncProgram {
         uint32_t can_make_work;
         uint32_t maker;
         uint32_t home;
-        unsigned long long int executed;
         unsigned long long int spills;
         unsigned long long int fills;
-        unsigned long long int made;
-        unsigned long long int idle;
-        unsigned long long int runs;
-        uint32_t idle_streak;
-        uint32_t make_pause;   // cycles before make_work is called again (pause_make_work)
-        uint32_t queue_check;  // cycles until the device queues are consulted again
         uint32_t share_check;  // cycles until the hungry count is consulted again
         uint32_t hungry;       // this group is counted in status->hungry
         uint32_t no_makers;    // this group has seen (and acquired) makers == 0
@@ -981,8 +990,7 @@ class AsyncProgram {
     // cycles, running its own links meanwhile, and pauses between cycles if
     // it has none. It stays a maker, so the launch does not end.
     __device__ HZ_INLINE void pause_make_work(unsigned int cycles) {
-        if (hz::wave::lane() == 0) _grp_ctx.make_pause = cycles;
-        hz::wave::sync();
+        _thd_ctx.make_pause = cycles;
     }
 
    protected:
@@ -1056,8 +1064,7 @@ class AsyncProgram {
     __device__ HZ_INLINE void make_work_step() {
         GroupContext& g = _grp_ctx;
         const uint32_t lane = hz::wave::lane();
-        if (lane == 0) g.made += 1;
-        hz::wave::sync();
+        _thd_ctx.made += 1u;
         if constexpr (LEVELS > 1) g.level = 0u;
         bool mine;
         [[clang::always_inline]] mine = PROGRAM_SPEC::make_work(*this);
@@ -1170,8 +1177,8 @@ class AsyncProgram {
             uint32_t rmask = hz::wave::first(g.ready_mask);
             g.free_mask = fm | (1u << L);
             g.ready_mask = rmask & ~(1u << L);
-            g.executed += cnt;
-            g.runs += 1;
+            _thd_ctx.executed += cnt;
+            _thd_ctx.runs += 1u;
             if constexpr (LEVELS > 1) g.level = link_level_of(L);
             if constexpr (HAS_LAZY) {
                 // Start the loads of the link most likely to run next.
@@ -1289,32 +1296,27 @@ class AsyncProgram {
         }
 
         uint32_t rm = hz::wave::first(g.ready_mask);
-        const uint32_t pause = hz::wave::first(g.make_pause);
-        if (pause != 0u) {
-            if (hz::wave::lane() == 0) g.make_pause = pause - 1u;
-            hz::wave::sync();
-        }
+        const uint32_t pause = hz::wave::first(_thd_ctx.make_pause);
+        if (pause != 0u) _thd_ctx.make_pause = pause - 1u;
         if (rm == 0) {
             bool check = !hz::wave::first(g.can_make_work);
             if (!check) {
-                uint32_t countdown = hz::wave::first(g.queue_check);
+                uint32_t countdown = hz::wave::first(_thd_ctx.queue_check);
                 check = countdown == 0;
-                g.queue_check = check ? QUEUE_CHECK_PERIOD - 1 : countdown - 1;
-                hz::wave::sync();
+                _thd_ctx.queue_check = check ? QUEUE_CHECK_PERIOD - 1 : countdown - 1;
             }
             if (check) {
                 HZ_PHASE(PHASE_PULL);
                 if (queues_hint()) {
                     pull_links(1);
                     rm = hz::wave::first(g.ready_mask);
-                    g.queue_check = 0;  // keep draining while links remain
-                    hz::wave::sync();
+                    _thd_ctx.queue_check = 0;  // keep draining while links remain
                 }
             }
             // Make work while no link is full, bounded so open links still run.
             HZ_PHASE(PHASE_MAKE);
             for (uint32_t tries = 0; rm == 0 && tries < 4 && pause == 0u && hz::wave::first(g.can_make_work) &&
-                                     hz::wave::first(g.make_pause) == 0u;
+                                     hz::wave::first(_thd_ctx.make_pause) == 0u;
                  ++tries) {
                 uint32_t c = 0;
                 fullest_open(c);
@@ -1340,7 +1342,7 @@ class AsyncProgram {
         }
 
         if (L != hz::LINK_NONE) {
-            g.idle_streak = 0;
+            _thd_ctx.idle_streak = 0;
             set_hungry(false);
             share_links(L);
             run_link(L);
@@ -1367,7 +1369,7 @@ class AsyncProgram {
             return RUN;
         }
         set_hungry(true);
-        uint32_t streak = hz::wave::first(g.idle_streak);
+        uint32_t streak = hz::wave::first(_thd_ctx.idle_streak);
         if (streak >= IDLE_EXIT_POLLS) return DONE;
         // makers only falls to zero after every maker's last units increment
         // (its decrement is a release), so once a relaxed read of makers sees
@@ -1386,9 +1388,9 @@ class AsyncProgram {
             }
             units = hz::wave::atomic_load(&_dev_ctx.status->units);
             flags = hz::wave::atomic_load(&_dev_ctx.status->flags);
-            g.idle += 1;
-            g.idle_streak = streak + 1;
         }
+        _thd_ctx.idle += 1u;
+        _thd_ctx.idle_streak = streak + 1u;
         makers = hz::wave::first(makers);
         units = hz::wave::first(units);
         flags = hz::wave::first(flags);
@@ -1416,10 +1418,7 @@ class AsyncProgram {
             g.can_make_work = 1;
             g.maker = 1;
             g.home = blockIdx.x % _dev_ctx.queue_count;
-            g.executed = g.spills = g.fills = g.made = g.idle = g.runs = 0;
-            g.idle_streak = 0;
-            g.make_pause = 0;
-            g.queue_check = 0;
+            g.spills = g.fills = 0;
             g.share_check = 0;
             g.hungry = 0;
             g.no_makers = 0;
@@ -1435,6 +1434,13 @@ class AsyncProgram {
         _thd_ctx.thread_id = blockIdx.x * blockDim.x + threadIdx.x;
         _thd_ctx.rand_state = _thd_ctx.thread_id;
         _thd_ctx.appended = 0;
+        _thd_ctx.executed = 0;
+        _thd_ctx.runs = 0;
+        _thd_ctx.made = 0;
+        _thd_ctx.idle = 0;
+        _thd_ctx.idle_streak = 0;
+        _thd_ctx.queue_check = 0;
+        _thd_ctx.make_pause = 0;
     }
 
     // Leaves nothing in LDS: every stash link goes back to the arena
@@ -1482,13 +1488,13 @@ class AsyncProgram {
                 hz::wave::atomic_add_acqrel(&_dev_ctx.status->makers, -1);
             }
             hz::AsyncStatus* st = _dev_ctx.status;
-            hz::wave::atomic_add(&st->promises, g.executed);
-            hz::wave::atomic_add(&st->links_run, g.runs);
+            hz::wave::atomic_add(&st->promises, _thd_ctx.executed);
+            hz::wave::atomic_add(&st->links_run, _thd_ctx.runs);
             if (appended) hz::wave::atomic_add(&st->appended, appended);
             if (g.spills) hz::wave::atomic_add(&st->spills, g.spills);
             if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
-            if (g.made) hz::wave::atomic_add(&st->make_work_calls, g.made);
-            if (g.idle) hz::wave::atomic_add(&st->idle_polls, g.idle);
+            if (_thd_ctx.made) hz::wave::atomic_add(&st->make_work_calls, _thd_ctx.made);
+            if (_thd_ctx.idle) hz::wave::atomic_add(&st->idle_polls, _thd_ctx.idle);
 #ifdef HZ_PHASE_TIMING
             for (int i = 0; i < 8; ++i) hz::wave::atomic_add(&st->phase_cycles[i], g.phase[i]);
 #endif
