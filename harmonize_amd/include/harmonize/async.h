@@ -403,7 +403,9 @@ class AsyncProgram {
         uint32_t link_level[LEVELS > 1 ? LINKS : 1];  // depth level of each link's promises
         uint32_t seq;
         uint32_t level;                                // level of the promises being evaluated
-        uint32_t open[OPS > 0 ? OPS * LEVELS : 1];     // open link of (operation, level)
+        // Open link of (operation, level) in bits 0-7 and its promise count
+        // in bits 8-15 (one LDS read gives an append both), or LINK_NONE.
+        uint32_t open[OPS > 0 ? OPS * LEVELS : 1];
         uint32_t free_mask;
         uint32_t ready_mask;
         uint32_t spare[SPARES];
@@ -887,6 +889,13 @@ class AsyncProgram {
         return pulled;
     }
 
+    // Entries of GroupContext::open.
+    __device__ static HZ_INLINE uint32_t open_entry(uint32_t link, uint32_t count) { return link | (count << 8); }
+    __device__ static HZ_INLINE uint32_t open_link(uint32_t entry) {
+        return entry == hz::LINK_NONE ? hz::LINK_NONE : (entry & 0xFFu);
+    }
+    __device__ static HZ_INLINE uint32_t open_count(uint32_t entry) { return entry >> 8; }
+
     // Appends one promise per active lane to the open link of operation D at
     // depth level lv (the same for every active lane).
     __device__ HZ_INLINE void stash_append(uint32_t D, uint32_t lv, const uint32_t* words) {
@@ -895,19 +904,20 @@ class AsyncProgram {
         uint32_t n = hz::wave::popc(act);
         uint32_t r = hz::wave::rank(act);
         const uint32_t slot = D * LEVELS + lv;
-        uint32_t L = hz::wave::first(g.open[slot]);
-        uint32_t c = 0;
+        const uint32_t entry = hz::wave::first(g.open[slot]);
+        uint32_t L, c = 0;
         _thd_ctx.appended += 1u;
-        if (L == hz::LINK_NONE) {
+        if (entry == hz::LINK_NONE) {
             L = take_link();
             if (L == hz::LINK_NONE) return;
             g.link_op[L] = D;
             if constexpr (LEVELS > 1) g.link_level[L] = lv;
             g.link_count[L] = 0;
-            g.open[slot] = L;
+            g.open[slot] = open_entry(L, 0u);
             hz::wave::sync();
         } else {
-            c = hz::wave::first(g.link_count[L]);
+            L = open_link(entry);
+            c = open_count(entry);
         }
         uint32_t room = 64u - c;
         if (n < room) {
@@ -917,6 +927,7 @@ class AsyncProgram {
 #pragma unroll
             for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
             g.link_count[L] = c + n;
+            g.open[slot] = open_entry(L, c + n);
             hz::wave::sync();
             return;
         }
@@ -946,7 +957,7 @@ class AsyncProgram {
         g.link_op[L2] = D;
         if constexpr (LEVELS > 1) g.link_level[L2] = lv;
         g.link_count[L2] = rest;
-        g.open[slot] = L2;
+        g.open[slot] = open_entry(L2, rest);
         hz::wave::sync();
     }
 
@@ -1009,9 +1020,10 @@ class AsyncProgram {
         for (uint32_t li = 0; li < LEVELS && best == hz::LINK_NONE; ++li) {
             const uint32_t lv = level_in_order(li);
             for (uint32_t d = 0; d < OPS; ++d) {
-                uint32_t L = hz::wave::first(g.open[d * LEVELS + lv]);
-                if (L == hz::LINK_NONE) continue;
-                uint32_t c = hz::wave::first(g.link_count[L]);
+                const uint32_t entry = hz::wave::first(g.open[d * LEVELS + lv]);
+                if (entry == hz::LINK_NONE) continue;
+                const uint32_t L = open_link(entry);
+                const uint32_t c = open_count(entry);
                 if (c > best_count) {
                     best = L;
                     best_count = c;
@@ -1263,9 +1275,11 @@ class AsyncProgram {
             } else {
                 uint32_t best = 0;
                 for (uint32_t d = 0; d < OPS * LEVELS; ++d) {
-                    uint32_t o = hz::wave::first(g.open[d]);
-                    if (o == hz::LINK_NONE || o == keep) continue;
-                    uint32_t c = hz::wave::first(g.link_count[o]);
+                    const uint32_t entry = hz::wave::first(g.open[d]);
+                    if (entry == hz::LINK_NONE) continue;
+                    const uint32_t o = open_link(entry);
+                    if (o == keep) continue;
+                    const uint32_t c = open_count(entry);
                     if (c > best) {
                         best = c;
                         L = o;

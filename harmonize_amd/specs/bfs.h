@@ -33,6 +33,10 @@ constexpr uint32_t BFS_DEFAULT_SPLIT = 64u;
 // the entry points reject values outside it.
 constexpr uint32_t BFS_MAX_SPLIT = 1u << 24;
 constexpr int BFS_FANOUT = 16;
+// Edges a lane relaxes per round of the wavefront's cooperative expansion.
+#ifndef HZ_BFS_CHAINS
+#define HZ_BFS_CHAINS 4
+#endif
 
 // One graph of the batch (32 bytes).
 struct BfsJob {
@@ -114,13 +118,15 @@ __device__ inline void bfs_relax_wave(PROGRAM& prog, unsigned int job, const Bfs
     }
     row[me] = BfsRange{j.col + begin, j.level, start, job, next, 0u};
     hz::wave::sync();
-    // Four edges per lane per round keep four memory chains in flight.
-    for (unsigned int base = 0; base < total; base += 4u * n) {
-        int w[4], seen[4], before[4];
-        const BfsRange* owner[4];
-        bool has[4];
+    // HZ_BFS_CHAINS edges per lane per round keep that many memory chains in
+    // flight.
+    constexpr int CH = HZ_BFS_CHAINS;
+    for (unsigned int base = 0; base < total; base += (unsigned int)CH * n) {
+        int w[CH], seen[CH], before[CH];
+        const BfsRange* owner[CH];
+        bool has[CH];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < CH; ++k) {
             const unsigned int e = base + (unsigned int)k * n + me;
             has[k] = e < total;
             owner[k] = row;
@@ -138,14 +144,14 @@ __device__ inline void bfs_relax_wave(PROGRAM& prog, unsigned int job, const Bfs
             }
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) seen[k] = has[k] ? hz_load(&owner[k]->level[w[k]]) : 0;
+        for (int k = 0; k < CH; ++k) seen[k] = has[k] ? hz_load(&owner[k]->level[w[k]]) : 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < CH; ++k) {
             const int nx = owner[k]->next;
             before[k] = has[k] && seen[k] > nx ? atomicMin(&owner[k]->level[w[k]], nx) : nx;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < CH; ++k) {
             if (has[k] && before[k] > owner[k]->next) {
                 prog.template async_call_cast<BfsVisit>(1, Promise<BfsVisit>(owner[k]->job, w[k], owner[k]->next));
             }

@@ -3,7 +3,12 @@ persistent kernel through the doorbell ring. Every job's starts and step
 counts against the oracle; more jobs than the job table holds, and more than
 the ring holds, published while one launch runs; the smallest-first claim
 order; a launch that ends when idle and the next submission launching again;
-launches cut short by their lifetime, carried on by the next launch."""
+launches cut short by their lifetime, carried on by the next launch; bad
+requests refused; jobs waiting for item space, in order, and released
+outputs freeing it."""
+
+import ctypes
+import time
 
 import numpy as np
 import pytest
@@ -129,6 +134,47 @@ def test_open_loop_trace(hz):
             assert r["status"] == hz.JOB_DONE
             assert r["end_ms"] >= r["submit_ms"]
             check_job(res, r["id"], int(s), int(sd))
+        res.drain(DRAIN_MS)
+    finally:
+        res.close()
+
+
+def test_bad_requests_are_refused(hz):
+    with pytest.raises(hz.HarmonizeError):
+        hz.Resident(bits=41)
+    res = hz.Resident(pool_bytes=12 * 4096, keep_outputs=True)
+    try:
+        with pytest.raises(hz.HarmonizeError):
+            res.submit(0, 1)
+        with pytest.raises(hz.HarmonizeError):
+            res.submit(5000, 1)  # more items than the 4,096-item space holds
+        j = hz._job("photon", 10, 1)
+        i = ctypes.c_uint64()
+        assert hz.lib().hz_resident_submit(res.handle, ctypes.byref(j), ctypes.byref(i)) != 0
+        assert "Collatz" in hz.last_error()
+    finally:
+        res.close()
+
+
+def test_jobs_wait_for_item_space_and_release_frees_it(hz):
+    """An item space that holds one 3,000-item job at a time: three such jobs
+    run one after another, in order, each correct; a released job's outputs
+    are gone and its items serve the next job."""
+    res = hz.Resident(pool_bytes=12 * 4096, keep_outputs=True)
+    try:
+        ids = [res.submit(3000, 40 + k) for k in range(3)]
+        for k, i in enumerate(ids):
+            deadline = time.monotonic() + 60.0
+            while res.job(i)["status"] != hz.JOB_DONE:
+                assert time.monotonic() < deadline, f"job {i} did not finish"
+                res.poll()
+            check_job(res, i, 3000, 40 + k)
+            res.release(i)
+            with pytest.raises(hz.HarmonizeError):
+                res.output(i, 0)
+        starts = [res.job(i)["start_ms"] for i in ids]
+        ends = [res.job(i)["end_ms"] for i in ids]
+        assert starts[1] >= ends[0] and starts[2] >= ends[1]
         res.drain(DRAIN_MS)
     finally:
         res.close()

@@ -34,9 +34,10 @@ def test_tree_matches_closed_form(order, roots, fanout, height):
 @pytest.mark.parametrize("order", ORDERS)
 def test_tree_levels_past_the_stack_and_relaunches(order):
     # Heights beyond the 8 levels run at the deepest level; short launches
-    # park every level's links in the arena between launches.
+    # park every level's links in the arena between launches (64 groups x 4
+    # cycles x at most 8 links of 64 promises is half the run's 262,080).
     roots, fanout, height = 64, 2, 11
-    leaves, violations, st = hz.tree_run("async", order, roots, fanout, height, groups=64, cycles=8,
+    leaves, violations, st = hz.tree_run("async", order, roots, fanout, height, groups=64, cycles=4,
                                          max_launches=100000, flags=hz.RUN_AUDIT)
     assert st["launches"] > 1 and st["audits"] == st["launches"] and st["spills"] > 0
     assert (leaves == fanout ** height).all() and violations == 0
