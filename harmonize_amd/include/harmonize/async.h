@@ -1587,10 +1587,27 @@ class AsyncProgram {
         bool pending = !run_now;
         unsigned int ticket = 0u;
         if (pending) {
-            hz::wave::atomic_add(&b.count, 1u);
-            // One atomic per await; acquire pairs with the release that set
-            // RELEASED_BIT, after which the promise runs at once.
-            ticket = hz::wave::atomic_add_acqrel(&b.tickets[D], 1u);
+            // One atomic per await, or one per wavefront when every awaiting
+            // lane names the same barrier: the lanes then take consecutive
+            // tickets in lane order, as separate draws in that order would.
+            // Acquire pairs with the release that set RELEASED_BIT, after
+            // which the promise runs at once.
+            const uint64_t act = hz::wave::active();
+            const uint64_t addr = (uint64_t)(uintptr_t)&b.tickets[D];
+            if (hz::wave::ballot(addr == hz::wave::first(addr)) == act) {
+                const uint32_t r = hz::wave::rank(act);
+                unsigned int base = 0u;
+                if (r == 0u) {
+                    const unsigned int n = hz::wave::popc(act);
+                    hz::wave::atomic_add(&b.count, n);
+                    base = hz::wave::atomic_add_acqrel(&b.tickets[D], n);
+                }
+                base = hz::wave::first(base);
+                ticket = (base & BARRIER::RELEASED_BIT) ? base : base + r;
+            } else {
+                hz::wave::atomic_add(&b.count, 1u);
+                ticket = hz::wave::atomic_add_acqrel(&b.tickets[D], 1u);
+            }
             if (ticket & BARRIER::RELEASED_BIT) {
                 run_now = true;
                 pending = false;
