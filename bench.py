@@ -366,20 +366,22 @@ def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -
     return out
 
 
-def resident_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -> dict:
-    """Collatz jobs only (sizes log-uniform in [2^10, 2^22] starts below
-    2^32, per-job seeds), served two ways on the same trace: admitted into a
+def resident_sweep(hz, kind: str = "collatz", jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -> dict:
+    """Jobs of one kind (Collatz: sizes log-uniform in [2^10, 2^22] starts
+    below 2^32; photon: the same sizes in photons, default walk parameters;
+    per-job seeds), served two ways on the same trace: admitted into a
     running resident kernel (hz_resident_*, arrival order and smallest job
     first) and by the slot server (one launch per job or chunk on 3 streams).
     Peak throughput with every job arriving at once, then Poisson arrivals at
-    fractions of each server's own peak; also one 2^22-start job alone."""
+    fractions of each server's own peak; also one 2^22-item job alone."""
     rng = np.random.default_rng(1)
     sizes = np.rint(2.0 ** rng.uniform(10.0, 22.0, size=jobs)).astype(np.int64)
     seeds = rng.integers(1, 1 << 40, size=jobs)
     trace = list(zip(sizes.tolist(), seeds.tolist()))
     unit = np.random.default_rng(2).exponential(1.0, size=jobs)
-    out = {"workload": f"{jobs} Collatz jobs, sizes log-uniform in [2^10, 2^22] starts uniform in [1, 2^32), "
-                       f"per-job seeds; the same trace for every server and load"}
+    items = "starts uniform in [1, 2^32)" if kind == "collatz" else "photons, default walk parameters"
+    out = {"workload": f"{jobs} {kind} jobs, sizes log-uniform in [2^10, 2^22] {items}, per-job seeds; "
+                       f"the same trace for every server and load"}
 
     def sweep(srv, run):
         run(srv, [(1024, 5)] * 8, np.zeros(8))
@@ -405,12 +407,12 @@ def resident_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) 
         return srv.run_trace(tr, arrivals)
 
     def run_slots(srv, tr, arrivals):
-        return srv.run_trace([("collatz", s, sd) for s, sd in tr], arrivals)
+        return srv.run_trace([(kind, s, sd) for s, sd in tr], arrivals)
 
     for name, make, run in (
-        ("resident_arrival_order", lambda: hz.Resident(pool_bytes=24 << 30), run_resident),
-        ("resident_smallest_first", lambda: hz.Resident(pool_bytes=24 << 30, policy=hz.RESIDENT_SMALLEST_FIRST),
-         run_resident),
+        ("resident_arrival_order", lambda: hz.Resident(pool_bytes=24 << 30, kind=kind), run_resident),
+        ("resident_smallest_first",
+         lambda: hz.Resident(pool_bytes=24 << 30, policy=hz.RESIDENT_SMALLEST_FIRST, kind=kind), run_resident),
         ("slots", lambda: hz.Server(pool_bytes=24 << 30, slots=3), run_slots),
     ):
         srv = make()
@@ -636,6 +638,7 @@ def main() -> int:
     if not args.no_serve:
         out["serving"] = serving_sweep(hz)
         out["serving_resident"] = resident_sweep(hz)
+        out["serving_resident_photon"] = resident_sweep(hz, "photon")
 
     pl = hz.collatz_run("plain", steps_out, starts, COUNT)
     if not np.array_equal(steps_out.numpy(), steps_async):

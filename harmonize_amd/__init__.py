@@ -818,7 +818,7 @@ class ResidentConfig(ctypes.Structure):
         ("life_ms", ctypes.c_uint32),
         ("bits", ctypes.c_uint32),
         ("keep_outputs", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("kind", ctypes.c_uint32),
     ]
 
 
@@ -859,20 +859,22 @@ _SIGNATURES.update(
 
 
 class Resident:
-    """Collatz jobs admitted into a running persistent kernel (hz_resident_*)."""
+    """Collatz or photon-walk jobs admitted into a running persistent kernel
+    (hz_resident_*); `kind` is "collatz" or "photon"."""
 
     def __init__(self, pool_bytes: int = 0, arena_links: int = 0, groups: int = 0,
                  policy: int = RESIDENT_BY_ADMISSION, idle_us: int = 0, poll_us: int = 0, life_ms: int = 0,
-                 bits: int = 0, keep_outputs: bool = False):
+                 bits: int = 0, keep_outputs: bool = False, kind: str = "collatz"):
+        self.kind = kind
         cfg = ResidentConfig(pool_bytes, arena_links, groups, policy, idle_us, poll_us, life_ms, bits,
-                             1 if keep_outputs else 0, 0)
+                             1 if keep_outputs else 0, JOB_TYPES[kind])
         h = _VP()
         check(lib().hz_resident_create(ctypes.byref(cfg), ctypes.byref(h)), "hz_resident_create")
         self.handle = h.value
 
     def submit(self, size: int, seed: int) -> int:
         i = ctypes.c_uint64()
-        j = _job("collatz", size, seed)
+        j = _job(self.kind, size, seed)
         check(lib().hz_resident_submit(self.handle, ctypes.byref(j), ctypes.byref(i)), "hz_resident_submit")
         return i.value
 
@@ -892,7 +894,12 @@ class Resident:
 
     def output(self, i: int, which: int = 0) -> np.ndarray:
         n = self.job(i)["size"]
-        out = np.empty(n, dtype=np.uint32 if which == 0 else np.uint64)
+        if which == 0:
+            out = np.empty(n, dtype=np.uint32)
+        elif self.kind == "photon":
+            out = np.empty(64 * (1000 + PHOTON_EXTRA_SLOTS), dtype=np.float64)
+        else:
+            out = np.empty(n, dtype=np.uint64)
         check(lib().hz_resident_output(self.handle, i, which, out.ctypes.data, out.nbytes), "hz_resident_output")
         return out
 
@@ -907,10 +914,11 @@ class Resident:
         return out
 
     def run_trace(self, sizes_seeds, arrival_ms, timeout_ms: float = 600000.0) -> list:
-        """Replays (size, seed) Collatz jobs arriving at arrival_ms (sorted, ms
-        after the call); returns one result dict per job."""
+        """Replays (size, seed) jobs of the server's kind arriving at
+        arrival_ms (sorted, ms after the call); returns one result dict per
+        job."""
         n = len(sizes_seeds)
-        arr = (Job * n)(*[_job("collatz", s, sd) for s, sd in sizes_seeds])
+        arr = (Job * n)(*[_job(self.kind, s, sd) for s, sd in sizes_seeds])
         t = np.ascontiguousarray(arrival_ms, dtype=np.float64)
         res = (JobResult * n)()
         check(lib().hz_resident_run_trace(self.handle, arr, t.ctypes.data, n, timeout_ms, res),

@@ -1,7 +1,8 @@
-// Resident serving of the C ABI: Collatz jobs admitted into a running
-// persistent kernel through a doorbell ring, no kernel launch per job.
+// Resident serving of the C ABI: Collatz or photon-walk jobs admitted into a
+// running persistent kernel through a doorbell ring, no kernel launch per job.
 //
-// One AsyncProgram<ResidentCollatzSpec> instance holds the device. The host
+// One AsyncProgram instance of the server's kind (ResidentCollatzSpec or
+// ResidentPhotonSpec) holds the device. The host
 // gives each admitted job a job-table entry (at most JOBS in flight) and a
 // slice of the device pool for its outputs, writes its descriptor into the
 // host-coherent ring and advances the ring head. If no launch is running, it
@@ -18,8 +19,15 @@
 #include <vector>
 
 #include "common.h"
+#include "photon_state.h"
 #include "pool.h"
 #include "resident.h"
+
+// Collatz claims: 8 starts per lane while at least this
+// many of a job's starts are unclaimed, one at a time below it.
+#ifndef HZ_RESIDENT_SPREAD_BELOW
+#define HZ_RESIDENT_SPREAD_BELOW (1u << 22)
+#endif
 
 using hz::capi::check;
 using hz::capi::guarded;
@@ -27,8 +35,30 @@ using hz::capi::Pool;
 
 namespace {
 
-using RProg = AsyncProgram<ResidentCollatzSpec>;
+using RCollatz = AsyncProgram<ResidentCollatzSpec>;
+using RPhoton = AsyncProgram<ResidentPhotonSpec>;
 namespace R = hz::resident;
+
+constexpr uint32_t PHOTON_REPLICAS = 64;  // tally copies per photon job
+
+// The server's program, of either kind.
+struct Program {
+    virtual ~Program() = default;
+    virtual void init(uint32_t groups, hipStream_t stream) = 0;
+    virtual void exec(uint32_t groups, hipStream_t stream) = 0;
+    virtual const hz::AsyncStatus* status() const = 0;
+    virtual uint32_t resident_groups() const = 0;
+};
+
+template <typename P>
+struct ProgramOf final : Program {
+    typename P::Instance inst;
+    ProgramOf(uint64_t arena_links, const ResidentState& st) : inst(arena_links, st, 0, 64u) {}
+    void init(uint32_t groups, hipStream_t stream) override { ::init<P>(inst, groups, stream); }
+    void exec(uint32_t groups, hipStream_t stream) override { ::exec<P>(inst, groups, 0xFFFFFFFFu, stream); }
+    const hz::AsyncStatus* status() const override { return inst.status.data(); }
+    uint32_t resident_groups() const override { return (uint32_t)hz::capi::resident_groups_of<P>(); }
+};
 
 double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -55,6 +85,7 @@ struct RJob {
     uint64_t items = 0;          // items reserved (the size, rounded up)
     bool joined = false;  // published while a launch was running
     double submit_ms = 0, start_ms = 0, end_ms = 0;
+    std::vector<double> tally;  // photon jobs: the job's tally, copied out when it finished
 };
 
 }  // namespace
@@ -62,8 +93,11 @@ struct RJob {
 struct hz_resident {
     hz_resident_config cfg{};
     std::chrono::steady_clock::time_point t0;
-    unsigned long long* starts = nullptr;  // item space: starts
-    unsigned int* steps = nullptr;         // item space: step counts
+    uint32_t kind = HZ_JOB_COLLATZ;
+    unsigned long long* starts = nullptr;  // item space: Collatz starts
+    unsigned int* steps = nullptr;         // item space: Collatz step counts or photon event counts
+    double* tallies = nullptr;             // photon: one tally per job-table entry
+    uint64_t tally_stride = 0;             // doubles per tally
     uint64_t capacity = 0;                 // items
     std::unique_ptr<Pool> pool;            // carves the item space, in items
     hipStream_t stream = nullptr;       // the resident launches
@@ -73,7 +107,7 @@ struct hz_resident {
     ResidentDoorbell* bell_dev = nullptr;
     ResidentCore* core = nullptr;
     unsigned int* counters = nullptr;
-    std::unique_ptr<RProg::Instance> inst;
+    std::unique_ptr<Program> prog;
     uint32_t groups = 0;
     bool running = false;
     bool resume = false;     // the last launch ended before completion: exec without init
@@ -93,7 +127,7 @@ struct hz_resident {
             (void)hipStreamSynchronize(stream);
         }
         if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-        inst.reset();
+        prog.reset();
         if (pass_done) (void)hipEventDestroy(pass_done);
         if (stream) (void)hipStreamDestroy(stream);
         if (copy_stream) (void)hipStreamDestroy(copy_stream);
@@ -101,6 +135,7 @@ struct hz_resident {
         if (counters) (void)hipFree(counters);
         if (starts) (void)hipFree(starts);
         if (steps) (void)hipFree(steps);
+        if (tallies) (void)hipFree(tallies);
         if (bell) (void)hipHostFree(bell);
     }
 
@@ -123,16 +158,16 @@ struct hz_resident {
         check(q, "resident launch");
         running = false;
         hz::AsyncStatus st{};
-        check(hipMemcpyAsync(&st, inst->status.data(), sizeof(st), hipMemcpyDeviceToHost, stream),
+        check(hipMemcpyAsync(&st, prog->status(), sizeof(st), hipMemcpyDeviceToHost, stream),
               "resident status copy");
         unsigned int error = 0;
         check(hipMemcpyAsync(&error, &core->error, sizeof(error), hipMemcpyDeviceToHost, stream),
               "resident error copy");
         check(hipStreamSynchronize(stream), "resident status");
-        if (st.flags & RProg::ARENA_EXHAUSTED_FLAG) {
+        if (st.flags & RCollatz::ARENA_EXHAUSTED_FLAG) {
             throw std::runtime_error("resident: the link arena was exhausted (raise arena_links)");
         }
-        if (st.flags & RProg::QUEUE_STALL_FLAG) {
+        if (st.flags & RCollatz::QUEUE_STALL_FLAG) {
             throw std::runtime_error(std::string("resident: a bounded wait gave up (") + hz::stall_name(st.stall_site) +
                                      ")");
         }
@@ -142,7 +177,7 @@ struct hz_resident {
         totals.idle_polls += st.idle_polls;
         for (int i = 0; i < 8; ++i) totals.phase_cycles[i] += st.phase_cycles[i];
         if (error) throw std::runtime_error("resident: the kernel rejected a ring entry");
-        resume = (st.flags & RProg::COMPLETION_FLAG) == 0u;
+        resume = (st.flags & RCollatz::COMPLETION_FLAG) == 0u;
     }
 
     void reap_jobs() {
@@ -155,6 +190,15 @@ struct hz_resident {
             RJob& j = jobs[(size_t)it->second];
             j.status = HZ_JOB_DONE;
             j.end_ms = now_ms(t0);
+            if (kind == HZ_JOB_PHOTON) {
+                // The tally lives in the job's table entry, which the next job
+                // installed there zeroes: copy it out first.
+                j.tally.resize(tally_stride);
+                check(hipMemcpyAsync(j.tally.data(), tallies + (size_t)j.table * tally_stride, 8ull * tally_stride,
+                                     hipMemcpyDeviceToHost, copy_stream),
+                      "resident tally copy");
+                check(hipStreamSynchronize(copy_stream), "resident tally copy");
+            }
             table_free |= 1ull << j.table;
             if (!cfg.keep_outputs) release_memory(j);
             finished.push_back(j.id);
@@ -196,8 +240,8 @@ struct hz_resident {
         __atomic_store_n(&bell->stop, draining ? 1u : 0u, __ATOMIC_RELEASE);
         hipLaunchKernelGGL(resident_reset_kernel, dim3(1), dim3(64), 0, stream, core);
         check(hipGetLastError(), "resident reset launch");
-        if (!resume) init<RProg>(*inst, groups, stream);
-        exec<RProg>(*inst, groups, 0xFFFFFFFFu, stream);
+        if (!resume) prog->init(groups, stream);
+        prog->exec(groups, stream);
         check(hipGetLastError(), "resident launch");
         check(hipEventRecord(pass_done, stream), "resident event");
         running = true;
@@ -214,7 +258,10 @@ struct hz_resident {
     bool idle() const { return pending.empty() && in_flight.empty() && !running; }
 
     uint64_t submit(const hz_job& d, double submit_ms) {
-        if (d.type != HZ_JOB_COLLATZ) throw std::invalid_argument("hz_resident_submit: only Collatz jobs");
+        if (d.type != kind) {
+            throw std::invalid_argument(kind == HZ_JOB_PHOTON ? "hz_resident_submit: this server runs photon-walk jobs"
+                                                              : "hz_resident_submit: this server runs Collatz jobs");
+        }
         if (d.size == 0 || d.size > (1u << 26)) throw std::invalid_argument("hz_resident_submit: size must be in [1, 2^26]");
         RJob j;
         j.id = jobs.size();
@@ -240,7 +287,7 @@ struct hz_resident {
         r->submit_ms = j.submit_ms;
         r->start_ms = j.start_ms;
         r->end_ms = j.end_ms;
-        r->state_bytes = 12ull * j.items;
+        r->state_bytes = (kind == HZ_JOB_PHOTON ? 4ull : 12ull) * j.items;
         r->chunks = 1;
     }
 
@@ -288,6 +335,10 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         if (c.bits == 0) c.bits = 32;
         if (c.bits > 40) throw std::invalid_argument("hz_resident_create: bits must be in [1, 40]");
         if (c.policy > R::SMALLEST_FIRST) throw std::invalid_argument("hz_resident_create: unknown policy");
+        if (c.kind != HZ_JOB_COLLATZ && c.kind != HZ_JOB_PHOTON) {
+            throw std::invalid_argument("hz_resident_create: kind must be HZ_JOB_COLLATZ or HZ_JOB_PHOTON");
+        }
+        r->kind = c.kind;
         r->t0 = std::chrono::steady_clock::now();
 
         int device = 0, khz = 0;
@@ -305,14 +356,22 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         check(hipHostGetDevicePointer((void**)&r->bell_dev, r->bell, 0), "hipHostGetDevicePointer");
         check(hipMalloc((void**)&r->core, sizeof(ResidentCore)), "hipMalloc(resident core)");
         check(hipMalloc((void**)&r->counters, sizeof(unsigned int) * R::COUNTER_WORDS), "hipMalloc(counters)");
-        // The item space: 12 bytes per item (a u64 start and a u32 count),
-        // indexed by u32.
-        r->capacity = c.pool_bytes / 12ull;
+        // The item space, indexed by u32: 12 bytes per Collatz item (a u64
+        // start and a u32 count), 4 per photon (its event count).
+        const bool photon = r->kind == HZ_JOB_PHOTON;
+        r->capacity = c.pool_bytes / (photon ? 4ull : 12ull);
         if (r->capacity > 0xFFFFFF00ull) r->capacity = 0xFFFFFF00ull;
         if (r->capacity < 256) throw std::invalid_argument("hz_resident_create: pool_bytes too small");
-        check(hipMalloc((void**)&r->starts, 8ull * r->capacity), "hipMalloc(resident starts)");
-        check(hipMalloc((void**)&r->steps, 4ull * r->capacity), "hipMalloc(resident steps)");
+        if (!photon) check(hipMalloc((void**)&r->starts, 8ull * r->capacity), "hipMalloc(resident starts)");
+        check(hipMalloc((void**)&r->steps, 4ull * r->capacity), "hipMalloc(resident outputs)");
         r->pool = std::make_unique<Pool>(r->capacity);
+        hz_photon_params pp{};
+        hz_photon_defaults(&pp);
+        pp.replicas = PHOTON_REPLICAS;
+        if (photon) {
+            r->tally_stride = (uint64_t)PHOTON_REPLICAS * (pp.bins + HZ_PHOTON_EXTRA_SLOTS);
+            check(hipMalloc((void**)&r->tallies, 8ull * r->tally_stride * R::JOBS), "hipMalloc(resident tallies)");
+        }
         {
             auto host = std::make_unique<ResidentCore>();
             std::memset(host.get(), 0, sizeof(ResidentCore));
@@ -332,10 +391,25 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         st.life_ticks = per_us * 1000ull * c.life_ms;
         st.policy = c.policy;
         st.claim = 8u;
-        st.spread_below = 1u << 22;
+        st.spread_below = HZ_RESIDENT_SPREAD_BELOW;
         st.bits = c.bits;
-        r->inst = std::make_unique<RProg::Instance>(c.arena_links, st, 0, 64u);
-        r->groups = c.groups ? c.groups : (uint32_t)hz::capi::resident_groups_of<RProg>();
+        if (photon) {
+            // A photon's lineages run ~20 events: claims of HZ_PHOTON_CLAIM
+            // photons per lane until the last HZ_PHOTON_SPREAD_BELOW, as in
+            // the photon program, keep the claim atomics (one address per
+            // job) off the critical path.
+            st.claim = HZ_PHOTON_CLAIM;
+            st.spread_below = HZ_PHOTON_SPREAD_BELOW;
+            st.events = r->steps;
+            st.tallies = r->tallies;
+            st.tally_stride = (unsigned int)r->tally_stride;
+            st.replicas = PHOTON_REPLICAS;
+            st.photon = hz::photon_abi::make_state(pp, nullptr, nullptr, nullptr);
+            r->prog = std::make_unique<ProgramOf<RPhoton>>(c.arena_links, st);
+        } else {
+            r->prog = std::make_unique<ProgramOf<RCollatz>>(c.arena_links, st);
+        }
+        r->groups = c.groups ? c.groups : r->prog->resident_groups();
         check(hipDeviceSynchronize(), "resident setup");
         *out = r.release();
     });
@@ -393,8 +467,18 @@ extern "C" int hz_resident_output(hz_resident* r, uint64_t id, uint32_t which, v
         const uint64_t n = j.desc.size;
         const void* src = nullptr;
         uint64_t want = 0;
-        if (which == 0) src = r->steps_of(j), want = 4 * n;
-        else if (which == 1) src = r->starts_of(j), want = 8 * n;
+        if (which == 0) {
+            src = r->steps_of(j), want = 4 * n;  // step counts or event counts
+        } else if (which == 1 && r->kind == HZ_JOB_COLLATZ) {
+            src = r->starts_of(j), want = 8 * n;
+        } else if (which == 1 && r->kind == HZ_JOB_PHOTON) {
+            want = 8ull * j.tally.size();
+            if (!host || bytes != want) {
+                throw std::invalid_argument("hz_resident_output: buffer size must be " + std::to_string(want));
+            }
+            std::memcpy(host, j.tally.data(), want);
+            return;
+        }
         if (!src) throw std::invalid_argument("hz_resident_output: no such output");
         if (!host || bytes != want) {
             throw std::invalid_argument("hz_resident_output: buffer size must be " + std::to_string(want));
@@ -425,8 +509,9 @@ extern "C" int hz_resident_info(const hz_resident* r, hz_resident_info_t* info) 
         info->joined_running = r->joined;
         info->groups = r->groups;
         info->running = r->running ? 1u : 0u;
-        info->pool_bytes = 12ull * r->pool->size();
-        info->pool_used = 12ull * r->pool->used();
+        const unsigned long long item_bytes = r->kind == HZ_JOB_PHOTON ? 4ull : 12ull;
+        info->pool_bytes = item_bytes * r->pool->size();
+        info->pool_used = item_bytes * r->pool->used();
         info->promises = r->totals.promises;
         info->links_run = r->totals.links_run;
         info->make_work_calls = r->totals.make_work_calls;

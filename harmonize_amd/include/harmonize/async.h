@@ -995,6 +995,11 @@ class AsyncProgram {
         return hz::wave::first(_grp_ctx.level);
     }
 
+    // Links this group has pulled from the device queues in this launch. A
+    // specification that caches per-group data about the promises it holds
+    // can tell from it whether promises have arrived from elsewhere.
+    __device__ HZ_INLINE unsigned long long links_pulled() { return hz::wave::first((uint64_t)_grp_ctx.fills); }
+
     // Called from make_work (by the whole wavefront) when it found nothing
     // to start but more may come later (a resident program waiting for
     // jobs): the group skips make_work for the next `cycles` scheduling

@@ -232,66 +232,87 @@ __device__ inline void tally_add(const PhotonState& st, unsigned int slot, doubl
 
 }  // namespace photon_detail
 
+namespace photon_detail {
+
+// One event of photon p, as the configuration defines it. The caller
+// supplies where its results go:
+//  - tally(slot, value) books weight in a tally slot;
+//  - end(q) ends the lineage of q (q.pending counts its events not yet added
+//    to the photon's event total);
+//  - split(copies) is told, before any copy is queued, that the photon splits
+//    into `copies` lineages (copies > 1);
+//  - emit(c) queues copy c.
+template <typename TALLY, typename END, typename SPLIT, typename EMIT>
+__host__ __device__ inline void event(const PhotonState& st, Photon p, TALLY&& tally, END&& end, SPLIT&& split,
+                                      EMIT&& emit) {
+    p.pending = (unsigned short)(p.pending + 1u);
+
+    hz::philox::Ctr r = hz::philox::philox4x32_10(p.id, p.event, p.lineage, 0u, st.seed_lo, st.seed_hi);
+    double u = (double)((r.v[0] >> 8) + 1u) * (1.0 / 16777216.0);  // (0, 1]
+    float dist = (float)(-hz::det::det_log(u) * (double)st.mean_free_path);
+    float nx = p.x + dist * p.ux;
+    float ny = p.y + dist * p.uy;
+    float nz = p.z + dist * p.uz;
+
+    if (nz < 0.0f || nz > st.thickness) {
+        tally(st.bins + (nz < 0.0f ? PHOTON_REFLECTED : PHOTON_TRANSMITTED), (double)p.w);
+        end(p);
+        return;
+    }
+
+    unsigned int bin = (unsigned int)(nz * st.bin_scale);
+    if (bin >= st.bins) bin = st.bins - 1u;
+    float w = p.w * st.albedo;
+    tally(bin, (double)p.w - (double)w);  // exact difference
+
+    unsigned int region = (unsigned int)(nz * st.region_scale);
+    if (region >= st.regions) region = st.regions - 1u;
+    WindowResult win = weight_window(st, w, region, r.v[1]);
+    if (win.fate == WINDOW_DIE) {
+        tally(st.bins + PHOTON_ROULETTE_LOST, (double)w);
+        end(p);
+        return;
+    }
+    if (win.fate == WINDOW_SURVIVE) {
+        tally(st.bins + PHOTON_ROULETTE_GAINED, (double)win.w - (double)w);
+    }
+    const unsigned int copies = win.copies;
+    w = win.w;
+
+    if (p.event + 1u >= st.max_events) {
+        tally(st.bins + PHOTON_CAPPED, (double)w * (double)copies);
+        end(p);
+        return;
+    }
+
+    if (copies > 1u) split(copies);
+    for (unsigned int j = 0; j < copies; ++j) {
+        Photon c;
+        c.x = nx;
+        c.y = ny;
+        c.z = nz;
+        c.w = w;
+        c.id = p.id;
+        c.event = (unsigned short)(p.event + 1u);
+        isotropic(st, p, j, j == 0 ? &r.v[2] : nullptr, c.ux, c.uy, c.uz);
+        c.lineage = (j == 0) ? p.lineage : child_lineage(p.lineage, p.event, j);
+        c.pending = (j == 0) ? p.pending : (unsigned short)0;
+        emit(c);
+    }
+}
+
+}  // namespace photon_detail
+
 struct PhotonStep {
     using Type = void (*)(Photon p);
 
     template <typename PROGRAM>
     __device__ static void eval(PROGRAM prog, Photon p) {
         const PhotonState& st = prog.device;
-        using photon_detail::tally_add;
-        p.pending = (unsigned short)(p.pending + 1u);
-
-        hz::philox::Ctr r = hz::philox::philox4x32_10(p.id, p.event, p.lineage, 0u, st.seed_lo, st.seed_hi);
-        double u = (double)((r.v[0] >> 8) + 1u) * (1.0 / 16777216.0);  // (0, 1]
-        float dist = (float)(-hz::det::det_log(u) * (double)st.mean_free_path);
-        float nx = p.x + dist * p.ux;
-        float ny = p.y + dist * p.uy;
-        float nz = p.z + dist * p.uz;
-
-        if (nz < 0.0f || nz > st.thickness) {
-            tally_add<PROGRAM>(st, st.bins + (nz < 0.0f ? PHOTON_REFLECTED : PHOTON_TRANSMITTED), (double)p.w);
-            atomicAdd(&st.events[p.id], (unsigned int)p.pending);
-            return;
-        }
-
-        unsigned int bin = (unsigned int)(nz * st.bin_scale);
-        if (bin >= st.bins) bin = st.bins - 1u;
-        float w = p.w * st.albedo;
-        tally_add<PROGRAM>(st, bin, (double)p.w - (double)w);  // exact difference
-
-        unsigned int region = (unsigned int)(nz * st.region_scale);
-        if (region >= st.regions) region = st.regions - 1u;
-        photon_detail::WindowResult win = photon_detail::weight_window(st, w, region, r.v[1]);
-        if (win.fate == photon_detail::WINDOW_DIE) {
-            tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_LOST, (double)w);
-            atomicAdd(&st.events[p.id], (unsigned int)p.pending);
-            return;
-        }
-        if (win.fate == photon_detail::WINDOW_SURVIVE) {
-            tally_add<PROGRAM>(st, st.bins + PHOTON_ROULETTE_GAINED, (double)win.w - (double)w);
-        }
-        const unsigned int copies = win.copies;
-        w = win.w;
-
-        if (p.event + 1u >= st.max_events) {
-            tally_add<PROGRAM>(st, st.bins + PHOTON_CAPPED, (double)w * (double)copies);
-            atomicAdd(&st.events[p.id], (unsigned int)p.pending);
-            return;
-        }
-
-        for (unsigned int j = 0; j < copies; ++j) {
-            Photon c;
-            c.x = nx;
-            c.y = ny;
-            c.z = nz;
-            c.w = w;
-            c.id = p.id;
-            c.event = (unsigned short)(p.event + 1u);
-            photon_detail::isotropic(st, p, j, j == 0 ? &r.v[2] : nullptr, c.ux, c.uy, c.uz);
-            c.lineage = (j == 0) ? p.lineage : photon_detail::child_lineage(p.lineage, p.event, j);
-            c.pending = (j == 0) ? p.pending : (unsigned short)0;
-            prog.template async<PhotonStep>(c);
-        }
+        photon_detail::event(
+            st, p, [&](unsigned int slot, double value) { photon_detail::tally_add<PROGRAM>(st, slot, value); },
+            [&](const Photon& q) { atomicAdd(&st.events[q.id], (unsigned int)q.pending); }, [](unsigned int) {},
+            [&](const Photon& c) { prog.template async<PhotonStep>(c); });
     }
 };
 

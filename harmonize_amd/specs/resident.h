@@ -38,6 +38,7 @@
 
 #include "collatz.h"
 #include "philox.h"
+#include "photon.h"
 
 namespace hz {
 namespace resident {
@@ -131,18 +132,28 @@ struct ResidentState {
     unsigned int claim;        // items per lane and claim while a job has spread_below or more left
     unsigned int spread_below;
     unsigned int bits;         // Collatz starts are uniform in [1, 2^bits)
+    // Photon jobs: event counts by item (the item space), one tally of
+    // `replicas` x (bins + PHOTON_EXTRA_SLOTS) doubles per job-table entry
+    // (tally_stride doubles), and the walk's parameters (seed, tally, events
+    // and iterator of `photon` unused; each job has its own seed).
+    unsigned int* events;
+    double* tallies;
+    unsigned int tally_stride;
+    unsigned int replicas;
+    PhotonState photon;
 };
 
 // Per lane: items claimed and not yet started, ends counted but not yet
-// added to the job's counters, and when this lane last saw a housekeeping
-// pass (device wall clock).
+// added to the job's counters, the job whose results the wavefront holds in
+// LDS (the same in every lane; NO_JOB if none) and when this lane last saw a
+// housekeeping pass (device wall clock).
 struct ResidentClaim {
     unsigned int job;
     unsigned int next;
     unsigned int end;
     unsigned int pend_job;  // job of the pending ends
     unsigned int pend;      // ends not yet added to the `ended` counters
-    unsigned int pad;
+    unsigned int held;
     unsigned long long poll_seen;
 };
 
@@ -158,8 +169,13 @@ namespace resident {
 // waits until its earlier stores have completed (s_waitcnt), so whoever sees
 // the publishing store sees them too. Acquire and release orderings would add
 // an L2 invalidate or write-back to every claim and poll; none of the shared
-// data needs them.
-__device__ HZ_INLINE void stores_done() { __builtin_amdgcn_s_waitcnt(0); }
+// data needs them. The signal fences keep the compiler from moving memory
+// operations across the wait.
+__device__ HZ_INLINE void stores_done() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 template <typename T>
 __device__ HZ_INLINE T ld(const T* p) {
@@ -257,6 +273,7 @@ __device__ HZ_INLINE unsigned int job_state(const ResidentState& s, unsigned int
 
 // Installs ring entry k into its job table entry (the whole wavefront, under
 // the housekeeping lock). `n` is the active list's length.
+template <typename KIND>
 __device__ HZ_INLINE bool install(const ResidentState& s, unsigned int k, unsigned int n) {
     ResidentCore* core = s.core;
     const ResidentEntry* e = &s.bell->entries[k % RING];
@@ -287,6 +304,7 @@ __device__ HZ_INLINE bool install(const ResidentState& s, unsigned int k, unsign
         st(&J.seed, pair_at(2));
         st(&J.base, word_at(4));
     }
+    KIND::install(s, job);
     // Claims that see the job open see its fields and zeroed counters.
     stores_done();
     if (lane == 0) {
@@ -315,6 +333,7 @@ __device__ HZ_INLINE bool install(const ResidentState& s, unsigned int k, unsign
 // is one of the POLLERS, `poll_ticks` have passed since the last pass it saw
 // and no other wavefront holds the lock: completions, new entries, the idle
 // decision.
+template <typename KIND>
 __device__ HZ_INLINE void poll(const ResidentState& s, ResidentClaim& t) {
     if (hz_group_index() >= POLLERS) return;
     ResidentCore* core = s.core;
@@ -376,7 +395,7 @@ __device__ HZ_INLINE void poll(const ResidentState& s, ResidentClaim& t) {
     const unsigned int head = hz::wave::first(sys_ld(&s.bell->head));
     unsigned int consumed = hz::wave::first(ld(&core->consumed));
     while (consumed != head && n < JOBS) {
-        if (install(s, consumed, n)) n += 1u;
+        if (install<KIND>(s, consumed, n)) n += 1u;
         consumed += 1u;
         if (lane == 0) {
             st(&core->consumed, consumed);
@@ -404,9 +423,13 @@ __device__ HZ_INLINE void poll(const ResidentState& s, ResidentClaim& t) {
 // jobs' claim counters are read by the lanes in parallel (one round trip when
 // no job is open), then one atomic claims for the whole wavefront. Returns
 // false if nothing could be claimed; `stopping` receives the launch's stop
-// flag.
-__device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& stopping) {
+// flag, `claimed` the job claimed from (NO_JOB if none) and `bulk` whether
+// lanes took more than one item each.
+__device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& stopping, unsigned int& claimed,
+                                bool& bulk) {
     stopping = false;
+    claimed = NO_JOB;
+    bulk = false;
     const uint64_t need = hz::wave::ballot(t.next >= t.end);
     if (need == 0ull) return true;
     ResidentCore* core = s.core;
@@ -444,6 +467,8 @@ __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& 
         base = hz::wave::read_lane(base, lead);
         per = hz::wave::read_lane(per, lead);
         const unsigned int job = hz::wave::read_lane(slot, lead);
+        claimed = job;
+        bulk = per > 1u;
         if (t.next >= t.end) {
             const unsigned int r = hz::wave::rank(need) * per;
             t.job = job;
@@ -456,7 +481,15 @@ __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& 
 }
 
 // The spec hooks shared by the resident kinds. KIND supplies the operations
-// (OpSet) and launch(prog, job, item) for one claimed item.
+// (OpSet), STASH_SIZE, and hooks that the whole wavefront calls:
+//  - init_group(prog) when a group starts;
+//  - claimed(prog, job, bulk) after a claim for lanes without items: `job`
+//    is the job claimed from (NO_JOB if nothing could be claimed), `bulk`
+//    whether lanes took more than one item each;
+//  - install(state, job) when `job` is installed (under the housekeeping
+//    lock, before the job opens);
+//  - launch(prog, job, item) for one claimed item (per lane);
+//  - finalize(prog) when the group's launch ends.
 template <typename KIND>
 struct ResidentSpecT {
     typedef typename KIND::OpSet OpSet;
@@ -474,7 +507,9 @@ struct ResidentSpecT {
         prog.thread.end = 0u;
         prog.thread.pend_job = NO_JOB;
         prog.thread.pend = 0u;
+        prog.thread.held = NO_JOB;
         prog.thread.poll_seen = 0ull;
+        KIND::init_group(prog);
     }
 
     // Items claimed and not started are started now (a launch that ends at
@@ -486,6 +521,7 @@ struct ResidentSpecT {
             prog.thread.next += 1u;
         }
         flush_ends(prog.device, prog.thread);
+        KIND::finalize(prog);
     }
 
     // Housekeeping when due, a claim for lanes without items, then one item
@@ -497,9 +533,11 @@ struct ResidentSpecT {
     __device__ static bool make_work(PROGRAM prog) {
         const ResidentState& s = prog.device;
         flush_ends(s, prog.thread);
-        poll(s, prog.thread);
-        bool stopping = false;
-        const bool found = claim(s, prog.thread, stopping);
+        poll<KIND>(s, prog.thread);
+        bool stopping = false, bulk = false;
+        unsigned int claimed = NO_JOB;
+        const bool found = claim(s, prog.thread, stopping, claimed, bulk);
+        if (claimed != NO_JOB || !found) KIND::claimed(prog, claimed, bulk);
         if (!found && !stopping) prog.pause_make_work(CLAIM_PAUSE);
         if (prog.thread.next < prog.thread.end) {
             KIND::launch(prog, prog.thread.job, prog.thread.next);
@@ -577,6 +615,14 @@ struct ResidentCollatzKind {
     static const size_t STASH_SIZE = 7;
 
     template <typename PROGRAM>
+    __device__ static HZ_INLINE void init_group(PROGRAM&) {}
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void claimed(PROGRAM&, unsigned int, bool) {}
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void finalize(PROGRAM&) {}
+    __device__ static HZ_INLINE void install(const ResidentState&, unsigned int) {}
+
+    template <typename PROGRAM>
     __device__ static HZ_INLINE void launch(PROGRAM& prog, unsigned int job, unsigned int index) {
         ResidentJob& J = prog.device.core->jobs[job];
         const unsigned long long seed = hz::resident::ld(&J.seed);
@@ -590,3 +636,208 @@ struct ResidentCollatzKind {
 };
 
 using ResidentCollatzSpec = hz::resident::ResidentSpecT<ResidentCollatzKind>;
+
+// ------------------------------------------------------------------ photon walk
+// The serving photon job: the photon walk with the workload's default
+// parameters, `size` photons and the job's seed, computed by the same event
+// body as PhotonStep (photon_detail::event), so every event, tally and count
+// is the oracle's. A promise is a Photon whose id holds the job-table entry
+// in its top bits and the photon's index within its job below them.
+// Results go to:
+//  - events[base + photon] (the item space), zeroed when the photon starts;
+//  - the job's tally in `tallies` (replica by group), zeroed at install.
+// A split into c lineages adds c - 1 to the job's `created` counters; every
+// lineage's end counts one end.
+//
+// Tally deposits are FP64 atomics. A wavefront that claims a bulk of a job's
+// photons holds that job's tally in LDS: deposits of the job's events that
+// the wavefront runs go there, the others to the job's device replica. While
+// it holds the tally the wavefront counts as one more lineage of the job (one
+// added to `created` when it takes the tally, one end after it has added the
+// tally to the device replica and waited for those writes), so the job cannot
+// be reported before its LDS deposits are in memory. It gives the tally back
+// when a claim finds nothing, when it claims from another job, and when its
+// launch ends.
+//
+// Per event the job's seed and base are needed. They are cached per group
+// in LDS, tagged with links_pulled(): an entry is used only while the group
+// has pulled no link since it was filled, and the entry of a job the group
+// claims from is dropped. A group holds a job's promises only through its own
+// claims or pulled links, so a cached entry always belongs to the incarnation
+// of the job whose promises the group holds.
+
+constexpr unsigned int RESIDENT_PHOTON_ID_BITS = 26;
+constexpr unsigned int RESIDENT_PHOTON_ID_MASK = (1u << RESIDENT_PHOTON_ID_BITS) - 1u;
+constexpr unsigned int RESIDENT_CACHE_NONE = 0xFFFFFFFFu;
+
+struct ResidentJobCache {
+    unsigned long long seed;
+    unsigned int base;
+    unsigned int tag;  // links_pulled() when filled, or RESIDENT_CACHE_NONE
+};
+
+template <typename PROGRAM>
+__device__ HZ_INLINE ResidentJobCache* resident_job_cache() {
+    constexpr unsigned int WAVES = (unsigned int)(PROGRAM::WORK_GROUP_SIZE / 64);
+    __shared__ ResidentJobCache cache[WAVES][hz::resident::JOBS];
+    return cache[hz_block_thread() / 64u];
+}
+
+template <typename PROGRAM>
+__device__ HZ_INLINE ResidentJobCache resident_job_params(PROGRAM& prog, unsigned int job) {
+    ResidentJobCache* cache = resident_job_cache<PROGRAM>();
+    const unsigned int tag = (unsigned int)prog.links_pulled();
+    ResidentJobCache e = cache[job];
+    if (e.tag != tag) {
+        ResidentJob& J = prog.device.core->jobs[job];
+        e.seed = hz::resident::ld(&J.seed);
+        e.base = hz::resident::ld(&J.base);
+        e.tag = tag;
+        cache[job] = e;
+    }
+    return e;
+}
+
+template <typename PROGRAM>
+__device__ HZ_INLINE double* resident_photon_tally() {
+    constexpr unsigned int WAVES = (unsigned int)(PROGRAM::WORK_GROUP_SIZE / 64);
+    __shared__ double tally[WAVES][photon_detail::PHOTON_LDS_SLOTS];
+    return tally[hz_block_thread() / 64u];
+}
+
+__device__ HZ_INLINE double* resident_photon_replica(const ResidentState& s, unsigned int job) {
+    return s.tallies + (size_t)job * s.tally_stride +
+           (size_t)(hz_group_index() & (s.replicas - 1u)) * (s.photon.bins + PHOTON_EXTRA_SLOTS);
+}
+
+struct RPhotonStep {
+    using Type = void (*)(Photon p);
+
+    template <typename PROGRAM>
+    __device__ static void eval(PROGRAM prog, Photon p) {
+        const ResidentState& s = prog.device;
+        const unsigned int job = p.id >> RESIDENT_PHOTON_ID_BITS;
+        const ResidentJobCache e = resident_job_params(prog, job);
+        PhotonState st = s.photon;
+        st.seed_lo = (unsigned int)e.seed;
+        st.seed_hi = (unsigned int)(e.seed >> 32);
+        Photon q = p;
+        q.id = p.id & RESIDENT_PHOTON_ID_MASK;
+        double* tally = job == prog.thread.held ? resident_photon_tally<PROGRAM>() : resident_photon_replica(s, job);
+        photon_detail::event(
+            st, q, [&](unsigned int slot, double value) { atomicAdd(&tally[slot], value); },
+            [&](const Photon& r) {
+                atomicAdd(&s.events[e.base + r.id], (unsigned int)r.pending);
+                hz::resident::count_end(s, prog.thread, job);
+            },
+            [&](unsigned int copies) {
+                // No wait is needed before the copies are queued: an end of
+                // any of them is added to the `ended` counters only after a
+                // wait for this wavefront's earlier memory operations (the
+                // deferred flush), or after the release fence of the spill
+                // that hands them to another group.
+                __hip_atomic_fetch_add(hz::resident::counter(s, job, 1u, hz_group_index() % hz::resident::STRIPES),
+                                       copies - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            },
+            [&](const Photon& c) {
+                Photon d = c;
+                d.id = c.id | (job << RESIDENT_PHOTON_ID_BITS);
+                prog.template async<RPhotonStep>(d);
+            });
+    }
+};
+
+struct ResidentPhotonKind {
+    typedef OpUnion<RPhotonStep> OpSet;
+    static const size_t STASH_SIZE = HZ_PHOTON_STASH;
+
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void init_group(PROGRAM&) {
+        ResidentJobCache* cache = resident_job_cache<PROGRAM>();
+        cache[hz::wave::lane()].tag = RESIDENT_CACHE_NONE;  // JOBS == 64 lanes
+        hz::wave::sync();
+    }
+
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void claimed(PROGRAM& prog, unsigned int job, bool bulk) {
+        if (job != hz::resident::NO_JOB) {
+            if (hz::wave::lane() == 0) resident_job_cache<PROGRAM>()[job].tag = RESIDENT_CACHE_NONE;
+            hz::wave::sync();
+        }
+        const unsigned int held = prog.thread.held;
+        if (held == job) return;
+        if (held != hz::resident::NO_JOB) give_back(prog);
+        const bool fits = prog.device.photon.bins + PHOTON_EXTRA_SLOTS <= photon_detail::PHOTON_LDS_SLOTS;
+        if (job != hz::resident::NO_JOB && bulk && fits) take(prog, job);
+    }
+
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void finalize(PROGRAM& prog) {
+        if (prog.thread.held != hz::resident::NO_JOB) give_back(prog);
+    }
+
+    // Takes `job`'s tally into LDS (zeroed) and counts the holder as a lineage.
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void take(PROGRAM& prog, unsigned int job) {
+        const ResidentState& s = prog.device;
+        double* t = resident_photon_tally<PROGRAM>();
+        const unsigned int slots = s.photon.bins + PHOTON_EXTRA_SLOTS;
+        for (unsigned int i = hz::wave::lane(); i < slots; i += 64u) t[i] = 0.0;
+        if (hz::wave::lane() == 0) {
+            __hip_atomic_fetch_add(hz::resident::counter(s, job, 1u, hz_group_index() % hz::resident::STRIPES), 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        hz::wave::sync();
+        prog.thread.held = job;
+    }
+
+    // Adds the held tally to the job's device replica, waits for those
+    // writes, then ends the holder's lineage.
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void give_back(PROGRAM& prog) {
+        const ResidentState& s = prog.device;
+        const unsigned int job = prog.thread.held;
+        hz::wave::sync();
+        const double* t = resident_photon_tally<PROGRAM>();
+        double* rep = resident_photon_replica(s, job);
+        const unsigned int slots = s.photon.bins + PHOTON_EXTRA_SLOTS;
+        for (unsigned int i = hz::wave::lane(); i < slots; i += 64u) {
+            const double v = t[i];
+            if (v != 0.0) atomicAdd(&rep[i], v);
+        }
+        hz::resident::stores_done();
+        if (hz::wave::lane() == 0) {
+            __hip_atomic_fetch_add(hz::resident::counter(s, job, 0u, hz_group_index() % hz::resident::STRIPES), 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        prog.thread.held = hz::resident::NO_JOB;
+    }
+
+    // Zeroes the job's tally (the whole wavefront).
+    __device__ static HZ_INLINE void install(const ResidentState& s, unsigned int job) {
+        double* t = s.tallies + (size_t)job * s.tally_stride;
+        for (unsigned int i = hz::wave::lane(); i < s.tally_stride; i += 64u) hz::resident::st(&t[i], 0.0);
+    }
+
+    template <typename PROGRAM>
+    __device__ static HZ_INLINE void launch(PROGRAM& prog, unsigned int job, unsigned int index) {
+        ResidentJob& J = prog.device.core->jobs[job];
+        const unsigned int item = hz::resident::ld(&J.base) + index;
+        hz::resident::store_out(&prog.device.events[item], 0u);
+        Photon p;
+        p.x = 0.0f;
+        p.y = 0.0f;
+        p.z = 0.0f;
+        p.ux = 0.0f;
+        p.uy = 0.0f;
+        p.uz = 1.0f;
+        p.w = 1.0f;
+        p.id = (job << RESIDENT_PHOTON_ID_BITS) | index;
+        p.event = 0u;
+        p.lineage = 0u;
+        p.pending = 0u;
+        prog.template async<RPhotonStep>(p);
+    }
+};
+
+using ResidentPhotonSpec = hz::resident::ResidentSpecT<ResidentPhotonKind>;

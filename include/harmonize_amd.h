@@ -641,19 +641,26 @@ int hz_server_run_trace(hz_server* server, const hz_job* jobs, const double* arr
  * (done or failed) since the previous poll. */
 int hz_serve_submit(hz_server* server, const hz_job* job, uint64_t* id);
 int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, uint32_t* count);
-/* Resident serving: Collatz jobs admitted into a running persistent kernel.
+/* Resident serving: Collatz or photon-walk jobs admitted into a running
+ * persistent kernel.
  * The serving plan the reference's design calls for (a resident kernel fed by
  * a host doorbell; the reference itself has no serving loop): the host writes
  * each job's descriptor into a ring in host-coherent memory, a running
  * AsyncProgram launch installs it into its job table and claims the job's
  * items in priority order, and reports the job's completion in the ring; a
  * launch with no job for idle_us ends, and the next submission launches
- * again. Jobs are HZ_JOB_COLLATZ with starts uniform in [1, 2^bits) from the
- * seed (the serving loop's generator); outputs: 0 = step counts (u32),
- * 1 = starts (u64). At most 64 jobs are in flight; the rest wait in order. */
+ * again. A server runs jobs of one kind:
+ *   HZ_JOB_COLLATZ  starts uniform in [1, 2^bits) from the seed (the serving
+ *                   loop's generator); outputs 0 = step counts (u32[size]),
+ *                   1 = starts (u64[size]);
+ *   HZ_JOB_PHOTON   the photon walk with hz_photon_defaults except photons =
+ *                   size and seed, 64 tally replicas; outputs 0 = events per
+ *                   photon (u32[size]), 1 = the tally (f64[64 x (bins + 5)]).
+ * At most 64 jobs are in flight; the rest wait in order. */
 typedef struct hz_resident hz_resident;
 typedef struct hz_resident_config {
-    uint64_t pool_bytes;   /* device memory for job outputs (0 = 4 GiB) */
+    uint64_t pool_bytes;   /* device memory for job outputs, 12 (Collatz) or 4 (photon) bytes
+                              per item (0 = 4 GiB) */
     uint64_t arena_links;  /* link arena of the program (0 = 2^18) */
     uint32_t groups;       /* groups of a launch (0 = every group that fits on the device) */
     uint32_t policy;       /* claim order: HZ_RESIDENT_BY_ADMISSION (0) or HZ_RESIDENT_SMALLEST_FIRST */
@@ -662,7 +669,7 @@ typedef struct hz_resident_config {
     uint32_t life_ms;      /* a launch stops claiming after this long; the next one continues (0 = 10000) */
     uint32_t bits;         /* Collatz starts in [1, 2^bits), 1..40 (0 = 32) */
     uint32_t keep_outputs; /* 1: a finished job keeps its outputs until hz_resident_release */
-    uint32_t reserved;
+    uint32_t kind;         /* HZ_JOB_COLLATZ (0) or HZ_JOB_PHOTON */
 } hz_resident_config;
 #define HZ_RESIDENT_BY_ADMISSION 0u
 #define HZ_RESIDENT_SMALLEST_FIRST 1u

@@ -178,3 +178,72 @@ def test_jobs_wait_for_item_space_and_release_frees_it(hz):
         res.drain(DRAIN_MS)
     finally:
         res.close()
+
+
+# ------------------------------------------------------------------ photon jobs
+# Events per photon bit-exact; tallies (FP64 atomics, summed in any order)
+# within 1e-12 relative of the oracle's.
+
+RTOL = 1e-12
+
+
+def check_photon(res, i, size, seed):
+    p = O.photon_params(photons=size, seed=seed)
+    ref_t, ref_e, _ = O.photon_program(p, threads=0)
+    assert np.array_equal(res.output(i, 0), ref_e), (i, size, seed)
+    got = res.output(i, 1).reshape(-1, 1005).sum(axis=0)
+    scale = np.maximum(np.abs(got), np.abs(ref_t))
+    assert (np.abs(got - ref_t) <= RTOL * scale).all(), (i, size, seed)
+
+
+@pytest.mark.parametrize("size", [1, 20000, (1 << 19) + 3, 1 << 22])
+def test_photon_job_matches_oracle(hz, size):
+    res = hz.Resident(kind="photon", keep_outputs=True)
+    try:
+        seed = 0xF0700 + size
+        i = res.submit(size, seed)
+        res.drain(DRAIN_MS)
+        assert res.job(i)["status"] == hz.JOB_DONE
+        check_photon(res, i, size, seed)
+        assert res.info()["pool_bytes"] == 4 << 30  # 2^30 photons of 4 bytes
+    finally:
+        res.close()
+
+
+def test_photon_jobs_reuse_table_entries(hz):
+    """80 photon jobs at once: more than the 64 table entries, so entries
+    (and their tallies) are reused while the launch runs; splits and roulette
+    exercise the created/ended accounting of branching jobs."""
+    rng = np.random.default_rng(3)
+    sizes = np.rint(2.0 ** rng.uniform(4, 13, size=80)).astype(int)
+    seeds = rng.integers(1, 1 << 40, size=80)
+    res = hz.Resident(kind="photon", keep_outputs=True)
+    try:
+        ids = [res.submit(int(s), int(sd)) for s, sd in zip(sizes, seeds)]
+        res.drain(DRAIN_MS)
+        assert res.info()["joined_running"] > 0
+        for i, s, sd in zip(ids, sizes, seeds):
+            check_photon(res, i, int(s), int(sd))
+    finally:
+        res.close()
+
+
+def test_photon_open_loop_trace(hz):
+    rng = np.random.default_rng(12)
+    sizes = np.rint(2.0 ** rng.uniform(10, 17, size=30)).astype(int)
+    seeds = rng.integers(1, 1 << 40, size=30)
+    arrivals = np.cumsum(rng.exponential(0.3, size=30))
+    res = hz.Resident(kind="photon", keep_outputs=True)
+    try:
+        out = res.run_trace(list(zip(sizes.tolist(), seeds.tolist())), arrivals)
+        for r, s, sd in zip(out, sizes, seeds):
+            assert r["status"] == hz.JOB_DONE
+            check_photon(res, r["id"], int(s), int(sd))
+        with pytest.raises(hz.HarmonizeError):
+            res.submit(0, 1)
+        j = hz._job("collatz", 10, 1)
+        k = ctypes.c_uint64()
+        assert hz.lib().hz_resident_submit(res.handle, ctypes.byref(j), ctypes.byref(k)) != 0
+        res.drain(DRAIN_MS)
+    finally:
+        res.close()
