@@ -23,12 +23,6 @@
 #include "pool.h"
 #include "resident.h"
 
-// Collatz claims: 8 starts per lane while at least this
-// many of a job's starts are unclaimed, one at a time below it.
-#ifndef HZ_RESIDENT_SPREAD_BELOW
-#define HZ_RESIDENT_SPREAD_BELOW (1u << 22)
-#endif
-
 using hz::capi::check;
 using hz::capi::guarded;
 using hz::capi::Pool;
@@ -390,16 +384,8 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         st.idle_ticks = per_us * c.idle_us;
         st.life_ticks = per_us * 1000ull * c.life_ms;
         st.policy = c.policy;
-        st.claim = 8u;
-        st.spread_below = HZ_RESIDENT_SPREAD_BELOW;
         st.bits = c.bits;
         if (photon) {
-            // A photon's lineages run ~20 events: claims of HZ_PHOTON_CLAIM
-            // photons per lane until the last HZ_PHOTON_SPREAD_BELOW, as in
-            // the photon program, keep the claim atomics (one address per
-            // job) off the critical path.
-            st.claim = HZ_PHOTON_CLAIM;
-            st.spread_below = HZ_PHOTON_SPREAD_BELOW;
             st.events = r->steps;
             st.tallies = r->tallies;
             st.tally_stride = (unsigned int)r->tally_stride;

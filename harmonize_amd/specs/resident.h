@@ -129,8 +129,6 @@ struct ResidentState {
     unsigned long long idle_ticks;   // idle ticks before the launch ends
     unsigned long long life_ticks;   // ticks after which a launch stops claiming (0: none)
     unsigned int policy;       // hz::resident::BY_ADMISSION or SMALLEST_FIRST
-    unsigned int claim;        // items per lane and claim while a job has spread_below or more left
-    unsigned int spread_below;
     unsigned int bits;         // Collatz starts are uniform in [1, 2^bits)
     // Photon jobs: event counts by item (the item space), one tally of
     // `replicas` x (bins + PHOTON_EXTRA_SLOTS) doubles per job-table entry
@@ -421,10 +419,12 @@ __device__ HZ_INLINE void poll(const ResidentState& s, ResidentClaim& t) {
 // Gives each lane without items a share of one job's items, from the first
 // job in claim order with items left: the active list, the open set and the
 // jobs' claim counters are read by the lanes in parallel (one round trip when
-// no job is open), then one atomic claims for the whole wavefront. Returns
-// false if nothing could be claimed; `stopping` receives the launch's stop
-// flag, `claimed` the job claimed from (NO_JOB if none) and `bulk` whether
-// lanes took more than one item each.
+// no job is open), then one atomic claims for the whole wavefront, of
+// KIND::per_lane(left, lanes) items per lane (`left` items unclaimed, `lanes`
+// lanes in the launch). Returns false if nothing could be claimed;
+// `stopping` receives the launch's stop flag, `claimed` the job claimed from
+// (NO_JOB if none) and `bulk` whether lanes took more than one item each.
+template <typename KIND>
 __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& stopping, unsigned int& claimed,
                                 bool& bulk) {
     stopping = false;
@@ -452,7 +452,7 @@ __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& 
         avail &= avail - 1ull;
         unsigned int base = 0u, got = 0u, per = 1u;
         if (lane == lead) {
-            per = limit - seen >= s.spread_below ? s.claim : 1u;
+            per = KIND::per_lane(limit - seen, gridDim.x * blockDim.x);
             ResidentJob& J = core->jobs[slot];
             const unsigned int want = want_lanes * per;
             const unsigned int old = __hip_atomic_fetch_add(&J.next, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -482,6 +482,8 @@ __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& 
 
 // The spec hooks shared by the resident kinds. KIND supplies the operations
 // (OpSet), STASH_SIZE, and hooks that the whole wavefront calls:
+//  - per_lane(left, lanes): items per lane of a claim from a job with `left`
+//    items unclaimed, in a launch of `lanes` lanes (one lane calls it);
 //  - init_group(prog) when a group starts;
 //  - claimed(prog, job, bulk) after a claim for lanes without items: `job`
 //    is the job claimed from (NO_JOB if nothing could be claimed), `bulk`
@@ -536,7 +538,7 @@ struct ResidentSpecT {
         poll<KIND>(s, prog.thread);
         bool stopping = false, bulk = false;
         unsigned int claimed = NO_JOB;
-        const bool found = claim(s, prog.thread, stopping, claimed, bulk);
+        const bool found = claim<KIND>(s, prog.thread, stopping, claimed, bulk);
         if (claimed != NO_JOB || !found) KIND::claimed(prog, claimed, bulk);
         if (!found && !stopping) prog.pause_make_work(CLAIM_PAUSE);
         if (prog.thread.next < prog.thread.end) {
@@ -613,6 +615,16 @@ struct RCollatzOdd {
 struct ResidentCollatzKind {
     typedef OpUnion<RCollatzEven, RCollatzOdd> OpSet;
     static const size_t STASH_SIZE = 7;
+
+    // One share of what is left per lane of the launch, 1 to 8 starts: a
+    // large job is claimed in bulk, so claims (one atomic on the job's claim
+    // counter per wavefront) are few, and every lane still gets starts. A
+    // fixed 8 per lane above 2^17 starts left ran 2^18-2^20-start jobs 40%
+    // slower, since their starts then fill only part of the device's lanes.
+    __device__ static HZ_INLINE unsigned int per_lane(unsigned int left, unsigned int lanes) {
+        const unsigned int per = left / lanes;
+        return per < 1u ? 1u : (per > 8u ? 8u : per);
+    }
 
     template <typename PROGRAM>
     __device__ static HZ_INLINE void init_group(PROGRAM&) {}
@@ -750,6 +762,14 @@ struct RPhotonStep {
 struct ResidentPhotonKind {
     typedef OpUnion<RPhotonStep> OpSet;
     static const size_t STASH_SIZE = HZ_PHOTON_STASH;
+
+    // The photon program's claims: HZ_PHOTON_CLAIM photons per lane while
+    // HZ_PHOTON_SPREAD_BELOW or more are left, else one. Bulk claims also
+    // take the job's tally into LDS. (One share of what is left per lane, as
+    // for Collatz, lowers the photon serving peak by a quarter.)
+    __device__ static HZ_INLINE unsigned int per_lane(unsigned int left, unsigned int) {
+        return left >= HZ_PHOTON_SPREAD_BELOW ? HZ_PHOTON_CLAIM : 1u;
+    }
 
     template <typename PROGRAM>
     __device__ static HZ_INLINE void init_group(PROGRAM&) {
