@@ -349,6 +349,34 @@ class AsyncProgram {
     static_assert(LAZY_WORDS <= 16, "a LazyLoad value may span at most 16 words");
     static constexpr uint32_t SPARES = 4;
 
+    // Link forwarding. While a link is evaluated, each lane's first async
+    // call (at the caller's depth level) is held in the lane's registers
+    // instead of being appended to the stash. If afterwards every held
+    // promise names the same operation and at least FORWARD_MIN lanes hold
+    // one, those promises run at once as a link of their own, straight from
+    // registers, and so on for up to FORWARD_RUNS links; otherwise, and
+    // after the last forwarded run, the held promises are appended to the
+    // stash as usual. A full link of one operation whose promises each queue
+    // one promise of another (a Collatz odd step is always followed by an
+    // even one) thus skips the stash's append, bookkeeping and read for its
+    // successor link (Collatz 2^26: 58.9 -> 52.1 ms per launch). Order of
+    // evaluation is all that changes. Only flat programs (one depth level)
+    // without asynchronous argument loads forward, and a specification can
+    // turn it off (FORWARD_LINKS = 0).
+#ifndef HZ_FORWARD
+#define HZ_FORWARD 1
+#endif
+#ifndef HZ_FORWARD_MIN
+#define HZ_FORWARD_MIN 48
+#endif
+#ifndef HZ_FORWARD_RUNS
+#define HZ_FORWARD_RUNS 4
+#endif
+    static constexpr bool FORWARD = HZ_FORWARD != 0 && hz::spec::FORWARD_LINKSOr<PROGRAM_SPEC, 1>::value != 0 &&
+                                    LEVELS == 1 && !HAS_LAZY;
+    static constexpr uint32_t FORWARD_MIN = HZ_FORWARD_MIN;
+    static constexpr uint32_t FORWARD_RUNS = HZ_FORWARD_RUNS;
+
     static const unsigned int BAD_FUNC_ID_FLAG = 0x00000001;
     static const unsigned int STASH_FAIL_FLAG = 0x00000002;
     static const unsigned int ARENA_EXHAUSTED_FLAG = 0x00000008;
@@ -392,6 +420,12 @@ class AsyncProgram {
         uint32_t idle_streak;      // consecutive idle cycles
         uint32_t queue_check;      // cycles until the device queues are consulted again
         uint32_t make_pause;       // cycles before make_work is called again (pause_make_work)
+        // Link forwarding: whether async calls are held (uniform; set while
+        // a link is evaluated), and this lane's held promise (its operation,
+        // LINK_NONE if none, and its words).
+        uint32_t fwd_on;
+        uint32_t fwd_op;
+        uint32_t fwd_words[FORWARD ? WORDS : 1];
     };
 
     // Per-wavefront scheduler state, in LDS.
@@ -985,6 +1019,14 @@ class AsyncProgram {
         pu.template cast<TYPE>() = param_value;
         uint32_t words[WORDS];
         __builtin_memcpy(words, &pu, sizeof(PromiseUnionType));
+        if constexpr (FORWARD) {
+            if (depth_delta == 0 && hz::wave::first(_thd_ctx.fwd_on) != 0u && _thd_ctx.fwd_op == hz::LINK_NONE) {
+                _thd_ctx.fwd_op = D;
+#pragma unroll
+                for (uint32_t w = 0; w < WORDS; ++w) _thd_ctx.fwd_words[w] = words[w];
+                return;
+            }
+        }
         stash_append(D, child_level(depth_delta), words);
     }
 
@@ -1215,6 +1257,7 @@ class AsyncProgram {
         }
         {
             HZ_PHASE(PHASE_EVAL);
+            if constexpr (FORWARD) _thd_ctx.fwd_on = 1u;
             if (lane < cnt) {
                 PromiseUnionType pu;
                 __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
@@ -1224,7 +1267,51 @@ class AsyncProgram {
                 pu.template loose_eval<ProgramType>(*this, op);
             }
             hz::wave::sync();
+            if constexpr (FORWARD) run_forwarded();
         }
+    }
+
+    // Runs the promises the lanes hold in registers while they form a link
+    // of one operation of at least FORWARD_MIN lanes (at most FORWARD_RUNS
+    // such links), then appends what is still held to the stash.
+    __device__ HZ_INLINE void run_forwarded() {
+        for (uint32_t k = 0; k < FORWARD_RUNS; ++k) {
+            const uint64_t held = hz::wave::ballot(_thd_ctx.fwd_op != hz::LINK_NONE);
+            if (held == 0ull) break;
+            const uint32_t D = hz::wave::read_lane(_thd_ctx.fwd_op, (uint32_t)__builtin_ctzll(held));
+            const uint64_t same = hz::wave::ballot(_thd_ctx.fwd_op == D);
+            const uint32_t n = (uint32_t)__builtin_popcountll(same);
+            if (same != held || n < FORWARD_MIN) break;
+            uint32_t words[WORDS];
+#pragma unroll
+            for (uint32_t w = 0; w < WORDS; ++w) words[w] = _thd_ctx.fwd_words[w];
+            const bool mine = _thd_ctx.fwd_op != hz::LINK_NONE;
+            _thd_ctx.fwd_op = hz::LINK_NONE;
+            _thd_ctx.executed += n;
+            _thd_ctx.runs += 1u;
+            if (mine) {
+                _thd_ctx.appended += 1u;  // counted as written into a link, as an append would
+                PromiseUnionType pu;
+                __builtin_memcpy(&pu, words, sizeof(PromiseUnionType));
+                pu.template loose_eval<ProgramType>(*this, D);
+            }
+            hz::wave::sync();
+        }
+        _thd_ctx.fwd_on = 0u;
+        uint64_t held = hz::wave::ballot(_thd_ctx.fwd_op != hz::LINK_NONE);
+        while (held != 0ull) {
+            const uint32_t D = hz::wave::read_lane(_thd_ctx.fwd_op, (uint32_t)__builtin_ctzll(held));
+            const bool mine = _thd_ctx.fwd_op == D;
+            held &= ~hz::wave::ballot(mine);
+            if (mine) {
+                uint32_t words[WORDS];
+#pragma unroll
+                for (uint32_t w = 0; w < WORDS; ++w) words[w] = _thd_ctx.fwd_words[w];
+                stash_append(D, 0u, words);
+            }
+            hz::wave::sync();
+        }
+        _thd_ctx.fwd_op = hz::LINK_NONE;
     }
 
     // True if the group's home queue holds links (at some depth level).
@@ -1460,6 +1547,8 @@ class AsyncProgram {
         _thd_ctx.idle_streak = 0;
         _thd_ctx.queue_check = 0;
         _thd_ctx.make_pause = 0;
+        _thd_ctx.fwd_on = 0;
+        _thd_ctx.fwd_op = hz::LINK_NONE;
     }
 
     // Leaves nothing in LDS: every stash link goes back to the arena

@@ -60,6 +60,10 @@ HZ_CONST_OR(OLDEST_FIRST)
 // Nonzero, with STACK_SIZE depth levels: run and pull the shallowest level
 // first instead of the deepest. Not a reference constant.
 HZ_CONST_OR(SHALLOW_FIRST)
+// Zero: the async program type never forwards links (runs a link's
+// successor promises straight from registers); forwarding is on by default
+// for flat programs. Not a reference constant.
+HZ_CONST_OR(FORWARD_LINKS)
 #undef HZ_CONST_OR
 
 // The operation set a program type executes: the specification's OpSet, plus
