@@ -60,7 +60,7 @@ def algorithmic_bytes(promises: int, starts: int) -> int:
 # reads for a 1 GiB device-to-device copy (`bench.py --leg copy`): FETCH_SIZE
 # reads 512 MiB for the copy's 1 GiB of reads (factor 2.0) and WRITE_SIZE
 # reads 1 GiB for its 1 GiB of writes (factor 1.0).
-TRAFFIC_KIB = {"fetch": 1023258.6, "write": 2205690.8}
+TRAFFIC_KIB = {"fetch": 998907.4, "write": 2188414.0}
 TRAFFIC_FACTOR = {"fetch": 2.0, "write": 1.0}
 
 
@@ -72,7 +72,7 @@ def measured_traffic_bytes() -> int:
 # `rocprofv3 --pmc SQ_INSTS_SALU SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_BRANCH
 # SQ_WAVES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT` pass over
 # `bench.py --leg collatz_async` (totals over all wavefronts of the dispatch).
-ISSUE_COUNTS = {"salu": 2.741e10, "valu": 1.581e10, "lds": 6.593e9, "branch": 9.463e9}
+ISSUE_COUNTS = {"salu": 2.5397e10, "valu": 1.7834e10, "lds": 4.7681e9, "branch": 9.3311e9}
 CLOCK_GHZ = 2.4  # MI355X peak engine clock
 LDS_BYTES_PER_CLOCK_PER_CU = 128
 
@@ -552,10 +552,12 @@ def main() -> int:
     # clock, so the chip's peak is CUs x clock.
     salu_peak = cus * CLOCK_GHZ  # G SALU instructions/s
     salu_achieved = ISSUE_COUNTS["salu"] / max(1, launches) / per_launch_s / 1e9
-    # Beside it: the LDS bytes the stash moves (each promise is written into a
-    # link and read back out, 2 x 16 B) against 128 B per CU per clock, and
-    # the measured HBM traffic against the HBM peak.
-    lds_achieved = 2 * PROMISE_BYTES * promises / max(1, launches) / per_launch_s / 1e9
+    # Beside it: the LDS bytes the stash moves (each promise that passes
+    # through it is written into a link and read back out, 2 x 16 B; promises
+    # run by link forwarding stay in registers) against 128 B per CU per
+    # clock, and the measured HBM traffic against the HBM peak.
+    stashed = promises - runs[-1]["forwarded"]
+    lds_achieved = 2 * PROMISE_BYTES * stashed / max(1, launches) / per_launch_s / 1e9
     lds_peak = cus * LDS_BYTES_PER_CLOCK_PER_CU * CLOCK_GHZ
     hbm_achieved = measured_traffic_bytes() / per_launch_s / 1e9
     alg_equiv = algorithmic_bytes(promises, COUNT) / max(1, launches) / per_launch_s / 1e9
@@ -580,6 +582,7 @@ def main() -> int:
             "groups": runs[-1]["groups"],
             "arena_links": arena_links,
             "promises_per_step": promises,
+            "forwarded_per_step": runs[-1]["forwarded"],
             "launches_per_step": launches,
             "spills": runs[-1]["spills"],
             "fills": runs[-1]["fills"],
@@ -600,7 +603,8 @@ def main() -> int:
             "counts_per_launch": {k: v / max(1, launches) for k, v in ISSUE_COUNTS.items()},
             "counters": "profiles/collatz_async_issue_counters.md",
             "lds": {"achieved": lds_achieved, "peak": lds_peak, "unit": "GB/s", "frac": lds_achieved / lds_peak,
-                    "rule": "2 x 16 B per promise (stash write + read); peak 128 B per CU per clock"},
+                    "rule": "2 x 16 B per promise through the stash (write + read; forwarded promises skip "
+                            "it); peak 128 B per CU per clock"},
             "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": hbm_achieved / HBM_PEAK_GBPS, "rule": "measured traffic per launch / launch time"},
             "algorithmic_hbm_equivalent": {

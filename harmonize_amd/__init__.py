@@ -61,6 +61,7 @@ class RunStats(ctypes.Structure):
         ("phase_cycles", ctypes.c_uint64 * 8),
         ("links_run", ctypes.c_uint64),
         ("arena_links", ctypes.c_uint64),
+        ("forwarded", ctypes.c_uint64),
     ]
 
     PHASES = ("cycle", "spill", "pull", "make_work", "pick", "read", "eval", "idle")

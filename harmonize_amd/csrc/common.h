@@ -141,6 +141,7 @@ void run_async(typename P::Instance& inst, uint32_t groups, uint32_t cycles, uin
         stats->elapsed_ms = ms;
         for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = s.phase_cycles[i];
         stats->links_run = s.links_run;
+        stats->forwarded = s.forwarded;
         stats->audits = audits;
         stats->arena_links = s.claim < inst.arena_size ? s.claim : inst.arena_size;
     }

@@ -169,6 +169,7 @@ extern "C" int hz_collatz_run_noalloc(const uint64_t* d_starts, uint32_t count, 
             stats->elapsed_ms = ms;
             for (int i = 0; i < 8; ++i) stats->phase_cycles[i] = st.phase_cycles[i];
             stats->links_run = st.links_run;
+            stats->forwarded = st.forwarded;
         }
     });
 }

@@ -77,6 +77,9 @@ struct AsyncStatus {
     // barrier awaits into a barrier link. Every promise written is evaluated
     // exactly once, so appended = promises + those still queued or held.
     unsigned long long int appended;
+    // Promises run straight from registers by link forwarding (they are
+    // counted in `appended` and `promises` but never pass through a stash).
+    unsigned long long int forwarded;
     // Shader-clock cycles per scheduler phase, summed over groups; filled only
     // when the runtime is built with HZ_PHASE_TIMING (the counterpart of the
     // reference's HRM_TIME phase timers, preamble/mod.h:59-67).
@@ -426,6 +429,7 @@ class AsyncProgram {
         uint32_t fwd_on;
         uint32_t fwd_op;
         uint32_t fwd_words[FORWARD ? WORDS : 1];
+        unsigned long long forwarded;  // promises the group ran by forwarding (uniform)
     };
 
     // Per-wavefront scheduler state, in LDS.
@@ -1289,6 +1293,7 @@ class AsyncProgram {
             _thd_ctx.fwd_op = hz::LINK_NONE;
             _thd_ctx.executed += n;
             _thd_ctx.runs += 1u;
+            _thd_ctx.forwarded += n;
             if (mine) {
                 _thd_ctx.appended += 1u;  // counted as written into a link, as an append would
                 PromiseUnionType pu;
@@ -1549,6 +1554,7 @@ class AsyncProgram {
         _thd_ctx.make_pause = 0;
         _thd_ctx.fwd_on = 0;
         _thd_ctx.fwd_op = hz::LINK_NONE;
+        _thd_ctx.forwarded = 0;
     }
 
     // Leaves nothing in LDS: every stash link goes back to the arena
@@ -1599,6 +1605,7 @@ class AsyncProgram {
             hz::wave::atomic_add(&st->promises, _thd_ctx.executed);
             hz::wave::atomic_add(&st->links_run, _thd_ctx.runs);
             if (appended) hz::wave::atomic_add(&st->appended, appended);
+            if (_thd_ctx.forwarded) hz::wave::atomic_add(&st->forwarded, _thd_ctx.forwarded);
             if (g.spills) hz::wave::atomic_add(&st->spills, g.spills);
             if (g.fills) hz::wave::atomic_add(&st->fills, g.fills);
             if (_thd_ctx.made) hz::wave::atomic_add(&st->make_work_calls, _thd_ctx.made);
@@ -1934,6 +1941,7 @@ class AsyncProgram {
             st->promises = st->spills = st->fills = st->make_work_calls = st->launches = st->idle_polls = 0;
             st->links_run = 0;
             st->appended = 0;
+            st->forwarded = 0;
             for (int i = 0; i < 8; ++i) st->phase_cycles[i] = 0;
         }
     }

@@ -345,6 +345,8 @@ typedef struct hz_run_stats {
                                  evaluation (64 = perfect coalescing) */
     uint64_t arena_links;     /* async: arena links ever taken from the bump
                                  allocator (the high-water mark of the arena) */
+    uint64_t forwarded;       /* async: promises run straight from registers by
+                                 link forwarding, bypassing the LDS stash */
 } hz_run_stats;
 
 /* Runs the Collatz program over `count` starts (d_starts may be NULL: start
