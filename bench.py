@@ -77,6 +77,15 @@ CLOCK_GHZ = 2.4  # MI355X peak engine clock
 LDS_BYTES_PER_CLOCK_PER_CU = 128
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what: str) -> None:
+    """A line on stderr after each part of the run (stdout carries only the
+    result line)."""
+    print(f"bench: {what} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
 def cpu_model() -> str:
     """The host CPU's model name (from /proc/cpuinfo, as lscpu reports it)."""
     try:
@@ -363,6 +372,7 @@ def serving_sweep(hz, jobs: int = SERVE_JOBS, loads=(0.2, 0.4, 0.6, 0.8, 1.0)) -
             out[name] = entry
         finally:
             srv.close()
+        progress(f"serving mix, {name}")
     return out
 
 
@@ -383,9 +393,10 @@ def resident_sweep(hz, kind: str = "collatz", jobs: int = SERVE_JOBS, loads=(0.2
     out = {"workload": f"{jobs} {kind} jobs, sizes log-uniform in [2^10, 2^22] {items}, per-job seeds; "
                        f"the same trace for every server and load"}
 
-    def sweep(srv, run):
+    def sweep(srv, run, label):
         run(srv, [(1024, 5)] * 8, np.zeros(8))
         lone = [run(srv, [(1 << 22, 77 + k)], np.zeros(1))[0] for k in range(3)]
+        progress(f"{label}, lone jobs")
         res = run(srv, trace, np.zeros(jobs))
         span = max(r["end_ms"] for r in res) - min(r["submit_ms"] for r in res)
         peak = jobs / (span * 1e-3)
@@ -401,23 +412,25 @@ def resident_sweep(hz, kind: str = "collatz", jobs: int = SERVE_JOBS, loads=(0.2
                                    "achieved_jobs_per_s": jobs / (span * 1e-3),
                                    "p50_ms": float(np.percentile(lat, 50)),
                                    "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())})
+            progress(f"{label}, load {load}")
         return entry
 
     def run_resident(srv, tr, arrivals):
-        return srv.run_trace(tr, arrivals)
+        return srv.run_trace(tr, arrivals, timeout_ms=120000.0)
 
     def run_slots(srv, tr, arrivals):
         return srv.run_trace([(kind, s, sd) for s, sd in tr], arrivals)
 
-    for name, make, run in (
-        ("resident_arrival_order", lambda: hz.Resident(pool_bytes=24 << 30, kind=kind), run_resident),
-        ("resident_smallest_first",
-         lambda: hz.Resident(pool_bytes=24 << 30, policy=hz.RESIDENT_SMALLEST_FIRST, kind=kind), run_resident),
-        ("slots", lambda: hz.Server(pool_bytes=24 << 30, slots=3), run_slots),
-    ):
+    servers = [("resident_arrival_order", lambda: hz.Resident(pool_bytes=24 << 30, kind=kind), run_resident)]
+    if kind == "collatz":  # photon servers claim in arrival order only
+        servers.append(("resident_smallest_first",
+                        lambda: hz.Resident(pool_bytes=24 << 30, policy=hz.RESIDENT_SMALLEST_FIRST, kind=kind),
+                        run_resident))
+    servers.append(("slots", lambda: hz.Server(pool_bytes=24 << 30, slots=3), run_slots))
+    for name, make, run in servers:
         srv = make()
         try:
-            out[name] = sweep(srv, run)
+            out[name] = sweep(srv, run, f"{kind} serving, {name}")
             if name.startswith("resident"):
                 info = srv.info()
                 out[name]["launches"] = info["launches"]
@@ -425,6 +438,7 @@ def resident_sweep(hz, kind: str = "collatz", jobs: int = SERVE_JOBS, loads=(0.2
                 out[name]["jobs"] = info["jobs"]
         finally:
             srv.close()
+        progress(f"{kind} serving, {name}")
     return out
 
 
@@ -615,6 +629,7 @@ def main() -> int:
         },
     }
 
+    progress("Collatz AsyncProgram")
     if not args.no_event:
         ev = hz.collatz_run("event", steps_out, starts, COUNT, max_launches=1 << 20)
         if not np.array_equal(steps_out.numpy(), steps_async):
@@ -627,17 +642,22 @@ def main() -> int:
             "groups": ev["groups"],
         }
         out["async_vs_event"] = value / out["event"]["value"]
+        progress("Collatz EventProgram")
 
     if not args.no_photon:
         out["photon"] = photon_comparison(hz)
+        progress("photon walk")
 
     if not args.no_bfs:
         out["bfs"] = bfs_comparison(hz)
+        progress("BFS batch")
 
     if not args.no_futures:
         out["futures"] = futures_comparison(hz)
+        progress("futures tree")
     if not args.no_barrier:
         out["barrier"] = barrier_demo(hz)
+        progress("barrier demo")
 
     if not args.no_serve:
         out["serving"] = serving_sweep(hz)
@@ -655,6 +675,7 @@ def main() -> int:
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        progress("CPU baseline")
 
     print(json.dumps(out), flush=True)
     return 0

@@ -79,7 +79,6 @@ struct RJob {
     uint64_t items = 0;          // items reserved (the size, rounded up)
     bool joined = false;  // published while a launch was running
     double submit_ms = 0, start_ms = 0, end_ms = 0;
-    std::vector<double> tally;  // photon jobs: the job's tally, copied out when it finished
 };
 
 }  // namespace
@@ -89,9 +88,8 @@ struct hz_resident {
     std::chrono::steady_clock::time_point t0;
     uint32_t kind = HZ_JOB_COLLATZ;
     unsigned long long* starts = nullptr;  // item space: Collatz starts
-    unsigned int* steps = nullptr;         // item space: Collatz step counts or photon event counts
-    double* tallies = nullptr;             // photon: one tally per job-table entry
-    uint64_t tally_stride = 0;             // doubles per tally
+    unsigned int* steps = nullptr;         // item space: Collatz step counts, or photon tallies and event counts
+    uint64_t tally_stride = 0;             // photon: doubles per tally
     uint64_t capacity = 0;                 // items
     std::unique_ptr<Pool> pool;            // carves the item space, in items
     hipStream_t stream = nullptr;       // the resident launches
@@ -129,12 +127,15 @@ struct hz_resident {
         if (counters) (void)hipFree(counters);
         if (starts) (void)hipFree(starts);
         if (steps) (void)hipFree(steps);
-        if (tallies) (void)hipFree(tallies);
         if (bell) (void)hipHostFree(bell);
     }
 
+    // A photon job's allocation holds its tally (2 x tally_stride words),
+    // then its event counts; a Collatz job's, its starts and step counts.
+    uint64_t lead() const { return kind == HZ_JOB_PHOTON ? 2ull * tally_stride : 0ull; }
     unsigned long long* starts_of(const RJob& j) const { return starts + j.base; }
-    unsigned int* steps_of(const RJob& j) const { return steps + j.base; }
+    unsigned int* steps_of(const RJob& j) const { return steps + j.base + lead(); }
+    double* tally_of(const RJob& j) const { return reinterpret_cast<double*>(steps + j.base); }
 
     void release_memory(RJob& j) {
         if (j.base != Pool::NONE) {
@@ -184,15 +185,6 @@ struct hz_resident {
             RJob& j = jobs[(size_t)it->second];
             j.status = HZ_JOB_DONE;
             j.end_ms = now_ms(t0);
-            if (kind == HZ_JOB_PHOTON) {
-                // The tally lives in the job's table entry, which the next job
-                // installed there zeroes: copy it out first.
-                j.tally.resize(tally_stride);
-                check(hipMemcpyAsync(j.tally.data(), tallies + (size_t)j.table * tally_stride, 8ull * tally_stride,
-                                     hipMemcpyDeviceToHost, copy_stream),
-                      "resident tally copy");
-                check(hipStreamSynchronize(copy_stream), "resident tally copy");
-            }
             table_free |= 1ull << j.table;
             if (!cfg.keep_outputs) release_memory(j);
             finished.push_back(j.id);
@@ -218,7 +210,7 @@ struct hz_resident {
             e.job = j.table;
             e.size = j.desc.size;
             e.seed = j.desc.seed;
-            e.base = (unsigned int)j.base;
+            e.base = (unsigned int)(j.base + lead());
             __atomic_store_n(&bell->head, head + 1u, __ATOMIC_RELEASE);
             head += 1u;
             in_flight[j.entry] = j.id;
@@ -260,7 +252,7 @@ struct hz_resident {
         RJob j;
         j.id = jobs.size();
         j.desc = d;
-        j.items = hz::capi::pool_align(d.size);
+        j.items = hz::capi::pool_align(lead() + d.size);
         if (j.items > pool->size()) throw std::invalid_argument("hz_resident_submit: job larger than the item space");
         j.submit_ms = submit_ms;
         jobs.push_back(j);
@@ -329,6 +321,12 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         if (c.bits == 0) c.bits = 32;
         if (c.bits > 40) throw std::invalid_argument("hz_resident_create: bits must be in [1, 40]");
         if (c.policy > R::SMALLEST_FIRST) throw std::invalid_argument("hz_resident_create: unknown policy");
+        if (c.kind == HZ_JOB_PHOTON && c.policy == R::SMALLEST_FIRST) {
+            // A photon server claiming smallest job first once stopped making
+            // progress in a serving sweep, for a reason not yet found; the
+            // combination stays refused until it is.
+            throw std::invalid_argument("hz_resident_create: photon jobs are claimed in arrival order only");
+        }
         if (c.kind != HZ_JOB_COLLATZ && c.kind != HZ_JOB_PHOTON) {
             throw std::invalid_argument("hz_resident_create: kind must be HZ_JOB_COLLATZ or HZ_JOB_PHOTON");
         }
@@ -351,7 +349,8 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         check(hipMalloc((void**)&r->core, sizeof(ResidentCore)), "hipMalloc(resident core)");
         check(hipMalloc((void**)&r->counters, sizeof(unsigned int) * R::COUNTER_WORDS), "hipMalloc(counters)");
         // The item space, indexed by u32: 12 bytes per Collatz item (a u64
-        // start and a u32 count), 4 per photon (its event count).
+        // start and a u32 count); for photon jobs 4 bytes per photon (its
+        // event count) plus the job's tally, 8 x tally_stride bytes.
         const bool photon = r->kind == HZ_JOB_PHOTON;
         r->capacity = c.pool_bytes / (photon ? 4ull : 12ull);
         if (r->capacity > 0xFFFFFF00ull) r->capacity = 0xFFFFFF00ull;
@@ -362,10 +361,7 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         hz_photon_params pp{};
         hz_photon_defaults(&pp);
         pp.replicas = PHOTON_REPLICAS;
-        if (photon) {
-            r->tally_stride = (uint64_t)PHOTON_REPLICAS * (pp.bins + HZ_PHOTON_EXTRA_SLOTS);
-            check(hipMalloc((void**)&r->tallies, 8ull * r->tally_stride * R::JOBS), "hipMalloc(resident tallies)");
-        }
+        if (photon) r->tally_stride = (uint64_t)PHOTON_REPLICAS * (pp.bins + HZ_PHOTON_EXTRA_SLOTS);
         {
             auto host = std::make_unique<ResidentCore>();
             std::memset(host.get(), 0, sizeof(ResidentCore));
@@ -387,7 +383,6 @@ extern "C" int hz_resident_create(const hz_resident_config* config, hz_resident*
         st.bits = c.bits;
         if (photon) {
             st.events = r->steps;
-            st.tallies = r->tallies;
             st.tally_stride = (unsigned int)r->tally_stride;
             st.replicas = PHOTON_REPLICAS;
             st.photon = hz::photon_abi::make_state(pp, nullptr, nullptr, nullptr);
@@ -458,12 +453,7 @@ extern "C" int hz_resident_output(hz_resident* r, uint64_t id, uint32_t which, v
         } else if (which == 1 && r->kind == HZ_JOB_COLLATZ) {
             src = r->starts_of(j), want = 8 * n;
         } else if (which == 1 && r->kind == HZ_JOB_PHOTON) {
-            want = 8ull * j.tally.size();
-            if (!host || bytes != want) {
-                throw std::invalid_argument("hz_resident_output: buffer size must be " + std::to_string(want));
-            }
-            std::memcpy(host, j.tally.data(), want);
-            return;
+            src = r->tally_of(j), want = 8ull * r->tally_stride;
         }
         if (!src) throw std::invalid_argument("hz_resident_output: no such output");
         if (!host || bytes != want) {

@@ -130,12 +130,13 @@ struct ResidentState {
     unsigned long long life_ticks;   // ticks after which a launch stops claiming (0: none)
     unsigned int policy;       // hz::resident::BY_ADMISSION or SMALLEST_FIRST
     unsigned int bits;         // Collatz starts are uniform in [1, 2^bits)
-    // Photon jobs: event counts by item (the item space), one tally of
-    // `replicas` x (bins + PHOTON_EXTRA_SLOTS) doubles per job-table entry
-    // (tally_stride doubles), and the walk's parameters (seed, tally, events
-    // and iterator of `photon` unused; each job has its own seed).
+    // Photon jobs: the item space as u32 words, where a job's allocation
+    // holds its tally of `replicas` x (bins + PHOTON_EXTRA_SLOTS) doubles
+    // (tally_stride doubles, 2 x tally_stride words) followed by its event
+    // counts, which start at the job's base; and the walk's parameters (seed,
+    // tally, events and iterator of `photon` unused; each job has its own
+    // seed).
     unsigned int* events;
-    double* tallies;
     unsigned int tally_stride;
     unsigned int replicas;
     PhotonState photon;
@@ -302,7 +303,7 @@ __device__ HZ_INLINE bool install(const ResidentState& s, unsigned int k, unsign
         st(&J.seed, pair_at(2));
         st(&J.base, word_at(4));
     }
-    KIND::install(s, job);
+    KIND::install(s, word_at(4));
     // Claims that see the job open see its fields and zeroed counters.
     stores_done();
     if (lane == 0) {
@@ -488,8 +489,8 @@ __device__ HZ_INLINE bool claim(const ResidentState& s, ResidentClaim& t, bool& 
 //  - claimed(prog, job, bulk) after a claim for lanes without items: `job`
 //    is the job claimed from (NO_JOB if nothing could be claimed), `bulk`
 //    whether lanes took more than one item each;
-//  - install(state, job) when `job` is installed (under the housekeeping
-//    lock, before the job opens);
+//  - install(state, base) when a job whose items start at `base` is
+//    installed (under the housekeeping lock, before the job opens);
 //  - launch(prog, job, item) for one claimed item (per lane);
 //  - finalize(prog) when the group's launch ends.
 template <typename KIND>
@@ -657,7 +658,8 @@ using ResidentCollatzSpec = hz::resident::ResidentSpecT<ResidentCollatzKind>;
 // in its top bits and the photon's index within its job below them.
 // Results go to:
 //  - events[base + photon] (the item space), zeroed when the photon starts;
-//  - the job's tally in `tallies` (replica by group), zeroed at install.
+//  - the job's tally, just before its event counts in the item space
+//    (replica by group), zeroed at install.
 // A split into c lineages adds c - 1 to the job's `created` counters; every
 // lineage's end counts one end.
 //
@@ -717,8 +719,13 @@ __device__ HZ_INLINE double* resident_photon_tally() {
     return tally[hz_block_thread() / 64u];
 }
 
-__device__ HZ_INLINE double* resident_photon_replica(const ResidentState& s, unsigned int job) {
-    return s.tallies + (size_t)job * s.tally_stride +
+// The tally of the job whose event counts start at item `base`.
+__device__ HZ_INLINE double* resident_photon_job_tally(const ResidentState& s, unsigned int base) {
+    return reinterpret_cast<double*>(s.events + (base - 2u * s.tally_stride));
+}
+
+__device__ HZ_INLINE double* resident_photon_replica(const ResidentState& s, unsigned int base) {
+    return resident_photon_job_tally(s, base) +
            (size_t)(hz_group_index() & (s.replicas - 1u)) * (s.photon.bins + PHOTON_EXTRA_SLOTS);
 }
 
@@ -735,7 +742,7 @@ struct RPhotonStep {
         st.seed_hi = (unsigned int)(e.seed >> 32);
         Photon q = p;
         q.id = p.id & RESIDENT_PHOTON_ID_MASK;
-        double* tally = job == prog.thread.held ? resident_photon_tally<PROGRAM>() : resident_photon_replica(s, job);
+        double* tally = job == prog.thread.held ? resident_photon_tally<PROGRAM>() : resident_photon_replica(s, e.base);
         photon_detail::event(
             st, q, [&](unsigned int slot, double value) { atomicAdd(&tally[slot], value); },
             [&](const Photon& r) {
@@ -819,7 +826,7 @@ struct ResidentPhotonKind {
         const unsigned int job = prog.thread.held;
         hz::wave::sync();
         const double* t = resident_photon_tally<PROGRAM>();
-        double* rep = resident_photon_replica(s, job);
+        double* rep = resident_photon_replica(s, hz::wave::first(hz::resident::ld(&s.core->jobs[job].base)));
         const unsigned int slots = s.photon.bins + PHOTON_EXTRA_SLOTS;
         for (unsigned int i = hz::wave::lane(); i < slots; i += 64u) {
             const double v = t[i];
@@ -834,8 +841,8 @@ struct ResidentPhotonKind {
     }
 
     // Zeroes the job's tally (the whole wavefront).
-    __device__ static HZ_INLINE void install(const ResidentState& s, unsigned int job) {
-        double* t = s.tallies + (size_t)job * s.tally_stride;
+    __device__ static HZ_INLINE void install(const ResidentState& s, unsigned int base) {
+        double* t = resident_photon_job_tally(s, base);
         for (unsigned int i = hz::wave::lane(); i < s.tally_stride; i += 64u) hz::resident::st(&t[i], 0.0);
     }
 

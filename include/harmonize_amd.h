@@ -661,11 +661,13 @@ int hz_serve_poll(hz_server* server, uint64_t* finished_ids, uint32_t capacity, 
  * At most 64 jobs are in flight; the rest wait in order. */
 typedef struct hz_resident hz_resident;
 typedef struct hz_resident_config {
-    uint64_t pool_bytes;   /* device memory for job outputs, 12 (Collatz) or 4 (photon) bytes
-                              per item (0 = 4 GiB) */
+    uint64_t pool_bytes;   /* device memory for job outputs (0 = 4 GiB): 12 bytes per Collatz
+                              start; 4 per photon plus the job's tally (8 x 64 x (bins + 5)
+                              bytes) per photon job */
     uint64_t arena_links;  /* link arena of the program (0 = 2^18) */
     uint32_t groups;       /* groups of a launch (0 = every group that fits on the device) */
-    uint32_t policy;       /* claim order: HZ_RESIDENT_BY_ADMISSION (0) or HZ_RESIDENT_SMALLEST_FIRST */
+    uint32_t policy;       /* claim order: HZ_RESIDENT_BY_ADMISSION (0) or HZ_RESIDENT_SMALLEST_FIRST
+                              (Collatz only) */
     uint32_t idle_us;      /* a launch with no job this long ends (0 = 1000) */
     uint32_t poll_us;      /* the kernel's housekeeping period (0 = 2) */
     uint32_t life_ms;      /* a launch stops claiming after this long; the next one continues (0 = 10000) */

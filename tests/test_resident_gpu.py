@@ -228,6 +228,11 @@ def test_photon_jobs_reuse_table_entries(hz):
         res.close()
 
 
+def test_photon_smallest_first_is_refused(hz):
+    with pytest.raises(hz.HarmonizeError, match="arrival order"):
+        hz.Resident(kind="photon", policy=hz.RESIDENT_SMALLEST_FIRST)
+
+
 def test_photon_open_loop_trace(hz):
     rng = np.random.default_rng(12)
     sizes = np.rint(2.0 ** rng.uniform(10, 17, size=30)).astype(int)
