@@ -91,8 +91,26 @@ def test_async_relaunch_and_spill_path():
     assert np.array_equal(got, ref)
     assert st["promises"] == promises
     assert st["launches"] > 1 and st["spills"] > 0 and st["fills"] == st["spills"]
-    # the host audit of queues, arena and promise accounts after every launch
+    # the host audit of queues, arena and promise accounts after every launch,
+    # with link forwarding running promises straight from registers
     assert st["audits"] == st["launches"]
+    assert 0 < st["forwarded"] < st["promises"]
+
+
+def test_async_forwarding_keeps_results_and_counts():
+    """Link forwarding: a full link of odd steps is followed by a full link
+    of even steps, which runs from registers. Starts 1..2^20: step counts and
+    the promise count are the oracle's, and a share of the promises (at most
+    the even steps that follow an odd one, so under half of all steps) ran
+    forwarded."""
+    n = 1 << 20
+    host = np.arange(1, n + 1, dtype=np.uint64)
+    ref, promises = O.collatz_steps(host)
+    dev = hz.DeviceArray.from_numpy(host)
+    got, st = run("async", n, starts=dev, max_launches=100000)
+    assert np.array_equal(got, ref)
+    assert st["promises"] == promises
+    assert 0 < st["forwarded"] < st["promises"] // 2
 
 
 @pytest.mark.parametrize("queues", [1, 3, 16])

@@ -39,6 +39,7 @@ def test_async_sequential_starts(leaves, readers):
     starts = np.arange(1, leaves + 1, dtype=np.uint64)
     total, echo, forwarded, st = _run("async", starts, readers)
     _check_against_oracle(starts, readers, total, echo, forwarded, st)
+    assert st["forwarded"] == 0  # the futures specification turns link forwarding off
 
 
 def test_async_closed_form_2p20():
