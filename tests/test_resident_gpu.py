@@ -228,6 +228,29 @@ def test_photon_jobs_reuse_table_entries(hz):
         res.close()
 
 
+def test_photon_tally_lives_in_the_item_space(hz):
+    """A photon job's allocation holds its tally (64 x 1005 doubles = 128,640
+    words) before its event counts: a space of 100,000 words refuses even a
+    10-photon job; without keep_outputs a finished job's tally is released
+    with its event counts."""
+    res = hz.Resident(kind="photon", pool_bytes=4 * 100000)
+    try:
+        with pytest.raises(hz.HarmonizeError, match="larger than the item space"):
+            res.submit(10, 1)
+    finally:
+        res.close()
+    res = hz.Resident(kind="photon")
+    try:
+        i = res.submit(3000, 9)
+        res.drain(DRAIN_MS)
+        assert res.job(i)["status"] == hz.JOB_DONE
+        assert res.info()["pool_used"] == 0
+        with pytest.raises(hz.HarmonizeError, match="released"):
+            res.output(i, 1)
+    finally:
+        res.close()
+
+
 def test_photon_smallest_first_is_refused(hz):
     with pytest.raises(hz.HarmonizeError, match="arrival order"):
         hz.Resident(kind="photon", policy=hz.RESIDENT_SMALLEST_FIRST)
