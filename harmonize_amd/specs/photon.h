@@ -319,12 +319,14 @@ struct PhotonStep {
 // Stash links per group, and photon ids each thread claims from the shared
 // iterator at once. make_work launches one photon per thread per call from
 // the thread's claimed block, so the shared counter (an atomic that every
-// group hits) is touched once per HZ_PHOTON_CLAIM calls.
+// group hits) is touched once per HZ_PHOTON_CLAIM calls. Claims of 4 took
+// the 1e7-photon walk from 5.62-5.77 ms (8) to 5.54-5.63 ms; claims of 2,
+// to 7.0 ms.
 #ifndef HZ_PHOTON_STASH
 #define HZ_PHOTON_STASH 4
 #endif
 #ifndef HZ_PHOTON_CLAIM
-#define HZ_PHOTON_CLAIM 8
+#define HZ_PHOTON_CLAIM 4
 #endif
 // Below this many unclaimed photons a thread claims one at a time, so a
 // small job (or the end of a large one) spreads its lineages over more
