@@ -375,6 +375,18 @@ class AsyncProgram {
 #ifndef HZ_FORWARD_RUNS
 #define HZ_FORWARD_RUNS 4
 #endif
+    // Layout of a link's promise words, the same in the stash and in the
+    // arena (so moving a link is a flat copy): promise-major when a promise
+    // is a whole number of 16-byte units, so that a lane writes and reads its
+    // promise with 16-byte accesses, and word-major otherwise.
+#ifndef HZ_PROMISE_MAJOR
+#define HZ_PROMISE_MAJOR 1
+#endif
+    static constexpr bool PROMISE_MAJOR = HZ_PROMISE_MAJOR != 0 && WORDS % 4u == 0u;
+    __device__ static HZ_INLINE uint32_t word_index(uint32_t promise, uint32_t word) {
+        return PROMISE_MAJOR ? promise * WORDS + word : word * 64u + promise;
+    }
+
     static constexpr bool FORWARD = HZ_FORWARD != 0 && hz::spec::FORWARD_LINKSOr<PROGRAM_SPEC, 1>::value != 0 &&
                                     LEVELS == 1 && !HAS_LAZY;
     static constexpr uint32_t FORWARD_MIN = HZ_FORWARD_MIN;
@@ -434,7 +446,7 @@ class AsyncProgram {
 
     // Per-wavefront scheduler state, in LDS.
     struct GroupContext {
-        uint32_t words[LINKS][WORDS][64];
+        alignas(16) uint32_t words[LINKS][WORDS * 64];  // word_index(promise, word)
         uint32_t link_op[LINKS];
         uint32_t link_count[LINKS];
         uint32_t link_seq[OLDEST_FIRST ? LINKS : 1];  // fill order of ready links (OLDEST_FIRST)
@@ -818,7 +830,7 @@ class AsyncProgram {
             return false;
         }
         unsigned int* dst = arena_link(id);
-        const uint32_t* src = &g.words[L][0][0];
+        const uint32_t* src = &g.words[L][0];
         if (n == 64) {
 #pragma unroll
             for (uint32_t w = 0; w < WORDS; ++w) dst[w * 64u + r] = src[w * 64u + r];
@@ -904,7 +916,7 @@ class AsyncProgram {
             uint32_t L = lowest(fm);
             const unsigned int* src = arena_link(id);
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][lane] = src[w * 64u + lane];
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w * 64u + lane] = src[w * 64u + lane];
             uint32_t meta = hz::wave::first(_dev_ctx.link_meta[id]);
             if (lane == 0) {
                 g.link_op[L] = meta & 0xFFu;
@@ -963,7 +975,7 @@ class AsyncProgram {
             // lane writes its promise and the (uniform) new count, with no
             // per-lane predication for the scalar unit to set up.
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][word_index(c + r, w)] = words[w];
             g.link_count[L] = c + n;
             g.open[slot] = open_entry(L, c + n);
             hz::wave::sync();
@@ -971,7 +983,7 @@ class AsyncProgram {
         }
         if (r < room) {
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][w][c + r] = words[w];
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L][word_index(c + r, w)] = words[w];
         }
         // L is now full: close it, then open another for the overflow.
         uint32_t rmask = hz::wave::first(g.ready_mask);
@@ -990,7 +1002,7 @@ class AsyncProgram {
         if (L2 == hz::LINK_NONE) return;
         if (r >= room) {
 #pragma unroll
-            for (uint32_t w = 0; w < WORDS; ++w) g.words[L2][w][r - room] = words[w];
+            for (uint32_t w = 0; w < WORDS; ++w) g.words[L2][word_index(r - room, w)] = words[w];
         }
         g.link_op[L2] = D;
         if constexpr (LEVELS > 1) g.link_level[L2] = lv;
@@ -1168,8 +1180,8 @@ class AsyncProgram {
                     const uint32_t lane = hz::wave::lane();
                     const uint32_t sw = lazy_source_word<I>();
                     if (lane < cnt) {
-                        const uint64_t lo = _grp_ctx.words[N][sw][lane];
-                        const uint64_t hi = _grp_ctx.words[N][sw + 1u][lane];
+                        const uint64_t lo = _grp_ctx.words[N][word_index(lane, sw)];
+                        const uint64_t hi = _grp_ctx.words[N][word_index(lane, sw + 1u)];
                         const uint32_t* src = (const uint32_t*)(uintptr_t)((hi << 32) | lo);
 #pragma unroll
                         for (uint32_t k = 0; k < LazyFacts<I>::VALUE_WORDS; ++k) {
@@ -1222,7 +1234,7 @@ class AsyncProgram {
             HZ_PHASE(PHASE_READ);
             if (lane < cnt) {
 #pragma unroll
-                for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][w][lane];
+                for (uint32_t w = 0; w < WORDS; ++w) words[w] = g.words[L][word_index(lane, w)];
             }
             if constexpr (HAS_LAZY) {
                 prefetched = hz::wave::first(g.pf_link) == L;
@@ -1836,7 +1848,7 @@ class AsyncProgram {
     __device__ HZ_INLINE void write_arena_slot(unsigned int L, unsigned int slot, const uint32_t* words) {
         unsigned int* dst = arena_link(L);
 #pragma unroll
-        for (uint32_t w = 0; w < WORDS; ++w) dst[w * 64u + slot] = words[w];
+        for (uint32_t w = 0; w < WORDS; ++w) dst[word_index(slot, w)] = words[w];
     }
 
     template <typename BARRIER>
