@@ -60,7 +60,7 @@ def algorithmic_bytes(promises: int, starts: int) -> int:
 # reads for a 1 GiB device-to-device copy (`bench.py --leg copy`): FETCH_SIZE
 # reads 512 MiB for the copy's 1 GiB of reads (factor 2.0) and WRITE_SIZE
 # reads 1 GiB for its 1 GiB of writes (factor 1.0).
-TRAFFIC_KIB = {"fetch": 1010638.1, "write": 2201102.9}
+TRAFFIC_KIB = {"fetch": 1011086.5, "write": 2198988.1}
 TRAFFIC_FACTOR = {"fetch": 2.0, "write": 1.0}
 
 
@@ -72,7 +72,7 @@ def measured_traffic_bytes() -> int:
 # `rocprofv3 --pmc SQ_INSTS_SALU SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_BRANCH
 # SQ_WAVES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE GRBM_COUNT` pass over
 # `bench.py --leg collatz_async` (totals over all wavefronts of the dispatch).
-ISSUE_COUNTS = {"salu": 2.4779e10, "valu": 1.8226e10, "lds": 4.1848e9, "branch": 8.9454e9}
+ISSUE_COUNTS = {"salu": 2.4486e10, "valu": 1.8018e10, "lds": 4.1635e9, "branch": 8.7764e9}
 CLOCK_GHZ = 2.4  # MI355X peak engine clock
 LDS_BYTES_PER_CLOCK_PER_CU = 128
 

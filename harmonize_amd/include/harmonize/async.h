@@ -313,10 +313,11 @@ class AsyncProgram {
     // Ready links a group may run per scheduling cycle. Running up to four
     // took Collatz 2^26 from 70.7 to 67.0 ms and the photon walk from 5.78 to
     // 5.66 ms, eight from 60.7 to 59.4 ms (with the group counters in
-    // registers); a program that shares work checks its hungry count once
-    // per cycle, so it runs one.
+    // registers), sixteen from 52.6-53.0 to 52.3-52.5 ms (with link
+    // forwarding; the photon walk, futures and barrier unchanged); a program
+    // that shares work checks its hungry count once per cycle, so it runs one.
 #ifndef HZ_BURST
-#define HZ_BURST 8
+#define HZ_BURST 16
 #endif
     static constexpr uint32_t BURST = SHARE_WORK ? 1u : (uint32_t)HZ_BURST;
 
