@@ -1741,7 +1741,12 @@ class AsyncProgram {
                 break;
             }
             if (pending) {
-                const unsigned long long seen = hz::wave::atomic_load_acquire(entry);
+                // Relaxed (coherent) polls: an acquire on every poll would
+                // invalidate the L2 each time (the barrier demo took 12.1-12.4
+                // instead of 10.6-11.1 ms at 1,024 barriers). The installer
+                // needs only the entry's value; a lane that finds its block's
+                // link acquires once before it writes into the link.
+                const unsigned long long seen = hz::wave::atomic_load(entry);
                 if (slot == 0u) {
                     // This lane installs the block's link once the block RING
                     // before it has filled and left the entry.
@@ -1763,6 +1768,9 @@ class AsyncProgram {
                         }
                     }
                 } else if ((unsigned int)(seen >> 32) == block && (unsigned int)seen != hz::BARRIER_NO_LINK) {
+                    // Pairs with the installer's release: the link's count
+                    // word and slot 0 are written.
+                    hz::wave::fence_acquire();
                     barrier_write(b, (unsigned int)seen, slot, D, words, block);
                     pending = false;
                     held = true;
