@@ -815,6 +815,17 @@ class AsyncProgram {
         return best_link;
     }
 
+    // Waits until this wavefront's earlier memory operations have completed:
+    // write-through stores are then at the device's coherence point, so a
+    // later relaxed store that publishes them needs no release fence (which
+    // would write back the XCD's whole L2). The signal fences keep the
+    // compiler from moving memory operations across the wait.
+    __device__ static HZ_INLINE void stores_done() {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+
     // Moves stash link L to the arena and queues it. Cooperative over the
     // active lanes (all 64 in the scheduler loop, a subset inside an async
     // call). Returns false if the arena is exhausted.
@@ -856,9 +867,7 @@ class AsyncProgram {
             __hip_atomic_store(&_dev_ctx.link_meta[id], op | (lv << 8) | (cnt << 16), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        stores_done();
         if (r == 0) {
             add_units(1);
             const uint32_t ring = ring_of(lv, g.home);
@@ -1773,7 +1782,8 @@ class AsyncProgram {
                             write_arena_slot(fresh, 0u, words);
                             // The count word and slot 0 are written before the
                             // block's other lanes read the link from the entry.
-                            hz::wave::atomic_store_release(entry, ((unsigned long long)block << 32) | fresh);
+                            stores_done();
+                            hz::wave::atomic_store(entry, ((unsigned long long)block << 32) | fresh);
                             barrier_finish(b, fresh, D, block);
                             held = true;
                         }
@@ -1836,7 +1846,10 @@ class AsyncProgram {
         for (uint32_t guard = 0; h < _dev_ctx.links && guard < _dev_ctx.links; ++guard) {
             unsigned int next = hz::wave::atomic_load(&_dev_ctx.link_aux[2u * h]);
             add_units(1);
-            queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), h);
+            // The link's slots and metadata were written through by the
+            // lanes that filled it.
+            stores_done();
+            queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), h, false);
             h = next;
         }
     }
@@ -1868,7 +1881,9 @@ class AsyncProgram {
     __device__ HZ_INLINE void write_arena_slot(unsigned int L, unsigned int slot, const uint32_t* words) {
         unsigned int* dst = arena_link(L);
 #pragma unroll
-        for (uint32_t w = 0; w < WORDS; ++w) dst[word_index(slot, w)] = words[w];
+        for (uint32_t w = 0; w < WORDS; ++w) {
+            __hip_atomic_store(&dst[word_index(slot, w)], words[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 
     template <typename BARRIER>
@@ -1881,8 +1896,10 @@ class AsyncProgram {
     // Counts one written slot; the lane that completes the link hands it on.
     template <typename BARRIER>
     __device__ HZ_INLINE void barrier_finish(BARRIER& b, unsigned int L, OpDisc d, unsigned int block) {
-        hz::wave::fence_release();
-        unsigned int before = hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], 1u);
+        // The slot was written through; once it has completed, the count
+        // that the link's completer reads covers it.
+        stores_done();
+        unsigned int before = hz::wave::atomic_add(&_dev_ctx.link_aux[2u * L + 1u], 1u);
         unsigned int filled = (before & 0xFFFFu) + 1u;
         unsigned int target = before >> 16;
         if (filled == target) barrier_complete(b, L, d, target, block);
@@ -1896,17 +1913,18 @@ class AsyncProgram {
                                                unsigned int block) {
         hz::wave::atomic_store(&b.ring[d][block % BARRIER::RING],
                                ((unsigned long long)block << 32) | hz::BARRIER_NO_LINK);
-        _dev_ctx.link_meta[L] = d | (count << 16);
-        hz::wave::fence_release();
+        hz::wave::atomic_store(&_dev_ctx.link_meta[L], (unsigned int)(d | (count << 16)));
+        stores_done();
         unsigned int h = hz::wave::atomic_load(&b.full_head);
         for (uint32_t spins = 0; spins < SLOT_SPIN_LIMIT; ++spins) {
             if (h == hz::BARRIER_RELEASED) {
                 add_units(1);
-                queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), L);
+                stores_done();
+                queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), L, false);
                 return;
             }
             hz::wave::atomic_store(&_dev_ctx.link_aux[2u * L], h);
-            hz::wave::fence_release();
+            stores_done();
             unsigned int seen = atomicCAS(&b.full_head, h, L);
             if (seen == h) return;
             h = seen;
