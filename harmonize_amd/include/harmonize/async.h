@@ -1731,15 +1731,10 @@ class AsyncProgram {
             if (hz::wave::ballot(addr == hz::wave::first(addr)) == act) {
                 const uint32_t r = hz::wave::rank(act);
                 unsigned int base = 0u;
-                if (r == 0u) {
-                    const unsigned int n = hz::wave::popc(act);
-                    hz::wave::atomic_add(&b.count, n);
-                    base = hz::wave::atomic_add_acqrel(&b.tickets[D], n);
-                }
+                if (r == 0u) base = hz::wave::atomic_add_acqrel(&b.tickets[D], hz::wave::popc(act));
                 base = hz::wave::first(base);
                 ticket = (base & BARRIER::RELEASED_BIT) ? base : base + r;
             } else {
-                hz::wave::atomic_add(&b.count, 1u);
                 ticket = hz::wave::atomic_add_acqrel(&b.tickets[D], 1u);
             }
             if (ticket & BARRIER::RELEASED_BIT) {
@@ -1841,15 +1836,41 @@ class AsyncProgram {
                 hz::wave::atomic_add_acqrel(&_dev_ctx.link_aux[2u * L + 1u], (part - 64u) << 16);
             if ((before & 0xFFFFu) == part) barrier_complete(b, L, d, part, block);
         }
-        unsigned int h = atomicExch(&b.full_head, hz::BARRIER_RELEASED);
+        const unsigned int head = atomicExch(&b.full_head, hz::BARRIER_RELEASED);
         hz::wave::fence_acquire();
-        for (uint32_t guard = 0; h < _dev_ctx.links && guard < _dev_ctx.links; ++guard) {
-            unsigned int next = hz::wave::atomic_load(&_dev_ctx.link_aux[2u * h]);
-            add_units(1);
-            // The link's slots and metadata were written through by the
-            // lanes that filled it.
-            stores_done();
-            queue_push(work_queue(_grp_ctx.home), work_slots(_grp_ctx.home), h, false);
+        // No link joins the list after the exchange, so it can be counted
+        // first: one unit update covers every link before the first is queued,
+        // and one tail reservation places them all. (A unit update waited for
+        // and a tail draw per link made the walk four round trips per link.)
+        unsigned int n = 0;
+        for (unsigned int L = head; L < _dev_ctx.links && n < _dev_ctx.links; ++n) {
+            L = hz::wave::atomic_load(&_dev_ctx.link_aux[2u * L]);
+        }
+        if (n == 0u) return;
+        add_units((int)n);
+        // The units, and the links' slots and metadata (written through by
+        // the lanes that filled them), are complete before any link is queued.
+        stores_done();
+        hz::LinkQueue* q = work_queue(_grp_ctx.home);
+        unsigned int* slots = work_slots(_grp_ctx.home);
+        const unsigned int t0 = hz::wave::atomic_add(&q->tail, n);
+        unsigned int h = head;
+        for (unsigned int i = 0; i < n; ++i) {
+            // Read before the link is queued: once run, it may be reused.
+            const unsigned int next = hz::wave::atomic_load(&_dev_ctx.link_aux[2u * h]);
+            unsigned int* slot = slots + ((t0 + i) & _dev_ctx.queue_mask);
+            uint32_t spins = 0;
+            while (hz::wave::atomic_load(slot) != hz::LINK_NONE) {
+                if (++spins > SLOT_SPIN_LIMIT) {
+                    stall(hz::STALL_QUEUE_PUSH, queue_detail(q, t0 + i));
+                    return;
+                }
+                hz::wave::sleep_short();
+            }
+            hz::wave::atomic_store(slot, h);
+            // Counted one by one, so that consumers free ring slots while a
+            // list longer than the ring is still being placed.
+            hz::wave::atomic_add(&q->count, 1);
             h = next;
         }
     }

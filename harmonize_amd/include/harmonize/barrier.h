@@ -56,11 +56,16 @@ struct RemappingBarrier {
     // Set in a ticket counter once the barrier is released.
     static constexpr unsigned int RELEASED_BIT = 0x80000000u;
 
+    // The three words every await or decrement touches sit on 128-byte lines
+    // of their own, so that the semaphore's decrements, the ticket draws and
+    // the full list's pushes of one barrier do not queue behind each other on
+    // one line. The semaphore stays first: a future's ReturnOp finds the
+    // barrier from its address.
     TaggedSemaphore semaphore;
-    unsigned int count;      // promises awaited on this barrier (statistic)
-    unsigned int full_head;  // list of full held links, or BARRIER_RELEASED
-    // Tickets handed out per operation (at most 2^31 before release).
-    unsigned int tickets[TYPE_COUNT > 0 ? TYPE_COUNT : 1];
+    alignas(128) unsigned int full_head;  // list of full held links, or BARRIER_RELEASED
+    // Tickets handed out per operation (at most 2^31 before release); their
+    // sum, without RELEASED_BIT, is the number of promises awaited so far.
+    alignas(128) unsigned int tickets[TYPE_COUNT > 0 ? TYPE_COUNT : 1];
     // ring[op][block % RING] = (block << 32) | link: the link of a block
     // being filled; (block << 32) | BARRIER_NO_LINK once it is full.
     unsigned long long int ring[TYPE_COUNT > 0 ? TYPE_COUNT : 1][RING];
@@ -72,7 +77,6 @@ struct RemappingBarrier {
     static HZ_HD RemappingBarrier blank(unsigned int sem_val) {
         RemappingBarrier b;
         b.semaphore = TaggedSemaphore(sem_val, hz::BARRIER_TAG);
-        b.count = 0u;
         b.full_head = hz::BARRIER_NO_LINK;
         for (size_t i = 0; i < (TYPE_COUNT > 0 ? TYPE_COUNT : 1); ++i) {
             b.tickets[i] = 0u;
@@ -82,6 +86,14 @@ struct RemappingBarrier {
             }
         }
         return b;
+    }
+
+    // Promises awaited on this barrier so far, released or held (a statistic,
+    // read when no await is in flight).
+    HZ_HD unsigned long long awaited() const {
+        unsigned long long n = 0;
+        for (size_t i = 0; i < (TYPE_COUNT > 0 ? TYPE_COUNT : 1); ++i) n += tickets[i] & ~RELEASED_BIT;
+        return n;
     }
 
     // Holds `promise` until release; runs it at once if already released
