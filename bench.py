@@ -213,7 +213,8 @@ def barrier_demo(hz, repeats: int = 3) -> dict:
     (counter = 2 * dependents + independents, demo.cpp:214), plus every held
     step run exactly once and none before its barrier's release. Measured with
     1,024 barriers (2,048 awaits each) and with 64 (32,768 awaits each), where
-    the awaits of one barrier contend on its partial-link word."""
+    each barrier's awaits share one ticket counter and its release queues 512
+    held links."""
 
     def best(barriers, reps):
         runs = []
